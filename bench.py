@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X SRK3 step path, in the reference's metric.
+
+Default workload (the headline configuration): Model-H-like compressible flow
+with N = 2 conserved phi4 scalars on a 1024 x 1024 active grid (dx = 1, 3/2
+padding to 1536^2), phi4 a = -1, kappa = 1, b = 1, M = 1, eta = zeta = 0.5,
+cT^2 = 1, rho0 = 1, advection on, SRK3.  The stated dt = 0.01 is beyond the
+explicit SRK3 stability bound at dx = 1, so dt = 0.005 is used.  Noise is on
+with kBT = 1e-3 (in-kernel Philox), so every stage draws its stochastic terms.
+
+`value` is active-grid Mpoint * SRK3-steps / s on one GPU with the state
+resident in HBM; `roofline` is the dominant kernel's algorithmic HBM bytes per
+launch over its mean launch time (HIP events on the solver's stream);
+`cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same
+workload on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "spectral-hohenberg-halperin-dynamics-2d_amd")
+sys.path.insert(0, PKG)
+
+import shhd  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name, precision):
+    if name == "model_h_1024":
+        na, n, dt = 1024, 2, 0.005
+        desc = "model-H compressible SRK3, N=2 phi4, 1024^2 active (3/2 -> 1536^2), dx=1, dt=0.005, kBT=1e-3"
+    elif name == "model_h_4096":
+        na, n, dt = 4096, 2, 0.005
+        desc = "model-H compressible SRK3, N=2 phi4, 4096^2 active (3/2 -> 6144^2), dx=1, dt=0.005, kBT=1e-3"
+    elif name == "ns_128":
+        na, n, dt = 128, 0, 0.05
+        desc = "compressible NS SRK3, N=0, 128^2 active (3/2 -> 192^2), dx=1, dt=0.05, kBT=1e-4"
+    elif name == "incompressible_4096":
+        cfg = shhd.SolverConfig(nx_active=4096, ny_active=4096, dealias="3/2", dt=1e-4,
+                                mode=shhd.MODE_INCOMPRESSIBLE, eta=1e-4, momentum_advection=True,
+                                noise_momentum=True, kBT=1e-8, seed=3, precision=precision)
+        return cfg, "incompressible SRK3, 4096^2 active (3/2 -> 6144^2), L=2pi, eta=1e-4, dt=1e-4, kBT=1e-8"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    if n == 0:
+        cfg = shhd.SolverConfig(nx_active=na, ny_active=na, dealias="3/2", lx=float(na), ly=float(na), dt=dt,
+                                pressure_coefficient=1.0, eta=0.1, zeta=0.1, momentum_advection=True,
+                                noise_momentum=True, kBT=1e-4, seed=1, precision=precision)
+    else:
+        cfg = shhd.SolverConfig(nx_active=na, ny_active=na, dealias="3/2", lx=float(na), ly=float(na), dt=dt,
+                                num_order_parameters=n, pressure_coefficient=1.0, eta=0.5, zeta=0.5,
+                                mobility=[1.0] * n, free_energy_kind=shhd.FREE_ENERGY_PHI4,
+                                mu_k0=[-1.0] * n, mu_k2=[1.0] * n, phi4_u=[1.0] * n,
+                                momentum_advection=True, order_parameter_advection=True,
+                                noise_momentum=True, noise_order_parameter=True, kBT=1e-3, seed=2,
+                                precision=precision)
+    return cfg, desc
+
+
+def initial_state(cfg, seed=2):
+    """rho = 1 + small fluctuations, psi ~ 0.01 Gaussian noise, small momentum;
+    Hermitian and masked (as every reference initial condition)."""
+    rng = np.random.default_rng(seed)
+    mask = shhd.active_mask(cfg)
+    m2 = cfg.nx * cfg.ny
+    st = np.zeros(cfg.state_shape(), np.complex128)
+    n_act = mask.sum()
+    for f in range(cfg.num_fields):
+        amp = 0.01 if 1 <= f <= cfg.num_order_parameters else 1e-3
+        z = (rng.standard_normal(mask.shape) + 1j * rng.standard_normal(mask.shape)) * mask
+        # physical rms ~ amp: |psi_k| ~ amp * Nc / sqrt(2 * active modes)
+        st[f] = z * (amp * m2 / math.sqrt(2.0 * n_act))
+    st[:, 0, 0] = 0
+    st[0, 0, 0] = m2
+    if cfg.mode == shhd.MODE_INCOMPRESSIBLE:
+        kx, ky, _ = shhd.wavenumbers(cfg)
+        k2 = kx ** 2 + ky ** 2
+        n = cfg.num_order_parameters
+        with np.errstate(invalid="ignore", divide="ignore"):
+            tr = np.where(k2 > 0, (-ky * st[n + 1] + kx * st[n + 2]) / k2, 0)
+        st[n + 1] = -ky * tr
+        st[n + 2] = kx * tr
+    return shhd.hermitian_project(st, cfg)
+
+
+def time_gpu(cfg, steps, warmup):
+    st = initial_state(cfg)
+    with shhd.GpuFluidSolver(cfg) as g:
+        g.set_state(st)
+        g.step(warmup, 0.0)
+        g.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        g.step(steps, warmup * cfg.dt)        # returns after the device finished
+        t1 = time.perf_counter()
+        kt = g.kernel_timing()
+        kb = g.kernel_bytes()
+        info = g.info()
+        out = g.get_state()
+    assert np.all(np.isfinite(out)), "non-finite state"
+    return t1 - t0, kt, kb, info
+
+
+def cpu_baseline(cfg_gpu, budget_s):
+    """FP64 CPU oracle (a restatement of the reference algorithm, own FFT,
+    OpenMP over rows/columns) on the same workload: 1 untimed step, then whole
+    steps until the time budget is spent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as op
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    cores = min(cores, 16)
+    used = op.set_threads(cores)
+    cfg = shhd.SolverConfig(**{**cfg_gpu.__dict__, "precision": 64})
+    o = op.Oracle(cfg)
+    o.set_state(initial_state(cfg))
+    o.step(1, 0.0)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1, (n + 1) * cfg.dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 50:
+            break
+    pts = cfg.nx_active * cfg.ny_active
+    return {"value": pts * n / el / 1e6, "unit": "Mpoint*SRK3-steps/s", "cores": used, "kind": "port",
+            "sample": f"{n} SRK3 steps of the same workload after 1 untimed step, {el:.1f} s, "
+                      f"FP64 oracle restatement, {used} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="model_h_1024")
+    ap.add_argument("--precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form side measurement")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every configuration of this path fits one MI355X and runs are "
+              "single-GPU only (see the sharding design in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, desc = workload(args.workload, args.precision)
+    el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup)
+    pts = cfg.nx_active * cfg.ny_active
+    value = pts * args.steps / el / 1e6
+
+    # dominant kernel class and its roofline
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms, n = kt[dom]
+    avg_s = ms / 1e3 / max(n, 1)
+    achieved = kb[dom] / avg_s / 1e9
+    kernels = {k: {"avg_ms": kt[k][0] / max(kt[k][1], 1), "launches": kt[k][1],
+                   "bytes_per_launch": kb[k], "GBs": kb[k] / (kt[k][0] / 1e3 / max(kt[k][1], 1)) / 1e9
+                   if kt[k][1] else None} for k in kt}
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(prof):
+        try:
+            t = json.load(open(prof))
+            traffic = t.get(args.workload, {}).get(f"fp{args.precision}", {}).get(dom)
+        except Exception:
+            traffic = None
+
+    result = {
+        "metric": "Mpoint*SRK3-steps/s",
+        "value": value,
+        "unit": "Mpoint*SRK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 64 else "f32",
+        "data": "synthetic (rho=1+1e-3 noise, psi ~ 1e-2 Gaussian, small momentum; seeded)",
+        "config": {"workload": desc, "grid": [cfg.nx_active, cfg.ny_active], "padded": [cfg.nx, cfg.ny],
+                   "order_parameters": cfg.num_order_parameters, "integrator": "srk3",
+                   "F_in": info.num_inverse_fields, "F_out": info.num_forward_fields,
+                   "F_s": info.num_updated_fields, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernels": kernels,
+        "step_GBs": sum(kb[k] * kt[k][1] for k in kt) / max(args.steps, 1) / (el / args.steps) / 1e9,
+    }
+    if args.precision == 64 and not args.no_fp32:
+        cfg32, _ = workload(args.workload, 32)
+        el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup)
+        v32 = pts * args.steps / el32 / 1e6
+        result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * shhd_gpu.h — C-ABI of the MI355X (gfx950) right-hand-side and time-step engine.
+ *
+ * The library replaces the reference solver's per-step hot path:
+ *   FluidSolver::step(State&, double t)          src/fluid_solver.h:97-99
+ *     -> TimeIntegrator::step(State&, t, RHSOperators)   src/time_integrator.h:141
+ *        -> FCalculator::{rho_det, psi_det, j_det, psi_sto, j_sto}
+ *           (src/fcalculator.h:64-69) and FCalculatorWorkspace
+ *           (src/fcalculator_workspace.cc:68-145), FourierTransform2D
+ *           (src/fourier_transform.h:39-51).
+ *
+ * Conventions shared with the reference:
+ *   - State layout (src/state.cc:8-38, src/domain.cc:21-30): fields ordered
+ *     [rho, psi_0 .. psi_{N-1}, jx, jy]; each field is complex[(Mx/2+1) * My],
+ *     index = gy * (Mx/2+1) + gx, interleaved (re, im) doubles.
+ *   - Wavenumbers (src/domain.cc:139-149): kx = 2*pi*gx/Lx,
+ *     ky = 2*pi*(gy <= My/2 ? gy : gy - My)/Ly.
+ *   - Active modes (src/spectral_mask.cc:45-52): gx < Nax/2 and |ky| < Nay/2.
+ *   - Forward 2D transform unscaled, backward scaled by 1/(Mx*My)
+ *     (src/fourier_transform.h:43,50).
+ *
+ * Ownership: host buffers belong to the caller; device memory belongs to the
+ * handle.  A handle is bound to one HIP stream and must be used from one thread
+ * at a time (the reference's FCalculator is likewise non-reentrant,
+ * src/fcalculator.h:35-38).  Errors cross the ABI as status codes; the message
+ * of the last failure is available from shhd_gpu_last_error().
+ */
+#ifndef SHHD_GPU_H
+#define SHHD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHHD_MAX_ORDER_PARAMETERS 8
+#define SHHD_MAX_FORCES 8
+
+typedef enum shhd_status {
+    SHHD_OK = 0,
+    SHHD_ERR_INVALID = 1,      /* invalid configuration or argument            */
+    SHHD_ERR_UNSUPPORTED = 2,  /* valid for the reference, no device functor   */
+    SHHD_ERR_HIP = 3,          /* HIP runtime error / device unavailable       */
+    SHHD_ERR_ZERO_DENSITY = 4, /* "compressible velocity encountered zero density." */
+    SHHD_ERR_RUNTIME = 5       /* other runtime error raised by the physics    */
+} shhd_status;
+
+/* dynamics mode: src/fcalculator_dynamics_mode.h:13-30 */
+enum { SHHD_MODE_COMPRESSIBLE = 0, SHHD_MODE_INCOMPRESSIBLE = 1, SHHD_MODE_QUIESCENT = 2 };
+/* integrator: src/fluid_solver.h:37-48 */
+enum { SHHD_INTEGRATOR_EULER = 0, SHHD_INTEGRATOR_SRK3 = 1 };
+/* free-energy style lowered to coefficients: src/model_free_energy*.cc */
+enum { SHHD_FREE_ENERGY_NULL = 0, SHHD_FREE_ENERGY_QUADRATIC = 1, SHHD_FREE_ENERGY_PHI4 = 2 };
+/* arithmetic of the device path */
+enum { SHHD_PRECISION_FP64 = 64, SHHD_PRECISION_FP32 = 32 };
+
+/* fix <ID> <momentum|order_parameter> force/sine on ... (src/simulationinfo.h:99-108) */
+typedef struct shhd_sine_force {
+    int momentum;          /* 1: momentum target, 0: order-parameter target   */
+    int component;         /* momentum: 0=x 1=y; order parameter: its index   */
+    int axis;              /* 0=x 1=y                                         */
+    int nk;                /* 0 < nk < active_N_axis/2                        */
+    double amplitude;
+} shhd_sine_force;
+
+/* fix <ID> order_parameter force/gradient on ... (src/simulationinfo.h:109-115) */
+typedef struct shhd_gradient_force {
+    int component;         /* order-parameter index                           */
+    int direction;         /* 0=x 1=y                                         */
+    double amplitude;
+} shhd_gradient_force;
+
+/*
+ * Everything the hot path reads from Params and the model plugins, lowered to
+ * plain numbers by the host (src/simulationinfo.h:35-173; the plugins'
+ * coefficient getters, src/model_free_energy.h:27-52,
+ * src/model_thermodynamics.h:25-27, src/model_transport_coefficient.h:11-85).
+ */
+typedef struct shhd_config {
+    int nx_active, ny_active;   /* `grid` (even)                               */
+    int nx, ny;                 /* compute grid after `dealias` (GridConfig)   */
+    double lx, ly;              /* `length`                                    */
+    int num_order_parameters;   /* N (0..SHHD_MAX_ORDER_PARAMETERS)            */
+    int mode;                   /* SHHD_MODE_*                                 */
+    int integrator;             /* SHHD_INTEGRATOR_*                           */
+    double dt;                  /* `timestep`                                  */
+    int precision;              /* SHHD_PRECISION_* (device arithmetic)        */
+
+    double pressure_coefficient;/* cT^2 of linear_eos, 0 for null thermo       */
+    double eta, zeta;           /* constant transport                          */
+    double mobility[SHHD_MAX_ORDER_PARAMETERS];   /* diagonal M[a,a]          */
+
+    int free_energy_kind;       /* SHHD_FREE_ENERGY_*                          */
+    double mu_k0[SHHD_MAX_ORDER_PARAMETERS];      /* mu_lin = (k0+k2 k^2+k4 k^4) psi */
+    double mu_k2[SHHD_MAX_ORDER_PARAMETERS];
+    double mu_k4[SHHD_MAX_ORDER_PARAMETERS];
+    double phi4_u[SHHD_MAX_ORDER_PARAMETERS];     /* mu_phys = u psi^3 (phi4)  */
+
+    int momentum_advection;          /* fix momentum nonlinear on            */
+    int order_parameter_advection;   /* fix order_parameter nonlinear on     */
+
+    int noise_momentum;              /* fix momentum noise on                */
+    int noise_order_parameter;       /* fix order_parameter noise on         */
+    double kBT;                      /* default 1.0                          */
+    double order_parameter_noise_chi;/* default 1.0                          */
+    uint64_t seed;                   /* default 12345                        */
+
+    int num_sine_forces;
+    shhd_sine_force sine_forces[SHHD_MAX_FORCES];
+    int num_gradient_forces;
+    shhd_gradient_force gradient_forces[SHHD_MAX_FORCES];
+} shhd_config;
+
+/* Static description of a created handle (no reference counterpart: it
+ * reports the transform counts F_in / F_out that size the byte model). */
+typedef struct shhd_info {
+    int num_fields;              /* N + 3                                      */
+    int num_inverse_fields;      /* F_in : fields inverse-transformed per stage*/
+    int num_forward_fields;      /* F_out: products forward-transformed        */
+    int num_updated_fields;      /* F_s  : state fields rewritten per stage    */
+    int stages_per_step;         /* 1 (Euler) or 3 (SRK3)                      */
+    size_t state_doubles;        /* doubles in the reference-layout state      */
+    size_t device_bytes;         /* device memory held by the handle           */
+} shhd_info;
+
+typedef struct shhd_gpu shhd_gpu;
+
+/* FluidSolver::FluidSolver (src/fluid_solver.h:52-95): builds the integrator
+ * and RHS engine.  Styles without a device functor fail here. */
+shhd_status shhd_gpu_create(const shhd_config* config, shhd_gpu** out);
+
+/* Upload a reference-layout state, (N+3)*(Mx/2+1)*My complex values given as
+ * n = 2*(N+3)*(Mx/2+1)*My doubles.  Inactive modes must be zero, as every
+ * reference initial condition writes active modes only
+ * (src/initial_condition_*.cc loops over SpectralMask2D::active_modes()). */
+shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* interleaved, size_t n);
+
+/* Download the state in the reference layout (inactive modes written as 0). */
+shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n);
+
+/* nsteps calls of FluidSolver::step(state, t0 + i*dt) (src/project.h:128-145,
+ * src/fluid_solver.h:97-99).  Returns after the device work has finished. */
+shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0);
+
+/* FluidSolver::set_flux_request / flux_buffer (src/fluid_solver.h:101-107).
+ * Flux recording has no device path yet: a request with any flag set returns
+ * SHHD_ERR_UNSUPPORTED; an all-zero request is accepted. */
+shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parameter, int momentum);
+shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n);
+
+/* Destroy the handle and free its device memory. */
+void shhd_gpu_destroy(shhd_gpu* h);
+
+/* Message of the last failing call on h (or of the last failing create when
+ * h is NULL). */
+const char* shhd_gpu_last_error(const shhd_gpu* h);
+
+/* ---- instrumentation (no reference counterpart) ---------------------------- */
+
+shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info);
+
+/* Per-kernel timing with HIP events recorded on the handle's stream.  While
+ * enabled, every kernel launch of shhd_gpu_step is bracketed by events; the
+ * accumulated time per kernel class (0: inverse column pass, 1: row pass,
+ * 2: forward column pass + stage update) is returned in ms together with the
+ * launch counts.  Enabling resets the accumulators. */
+shhd_status shhd_gpu_set_kernel_timing(shhd_gpu* h, int enable);
+shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long* launches, int nclasses);
+
+/* Algorithmic HBM bytes moved by one launch of each kernel class (the byte
+ * model: compact spectra s*Na^2 per field, column/row intermediates
+ * 1.5*s*Na^2 per field under 3/2 padding). */
+shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int nclasses);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHHD_GPU_H */

@@ -1,0 +1,141 @@
+"""ctypes binding of oracle/liboracle.so — the FP64 CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  Built by `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_PKG = os.path.join(os.path.dirname(HERE), "spectral-hohenberg-halperin-dynamics-2d_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+import shhd  # noqa: E402  (config struct shared with the device library)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle not built: run `make -C {HERE}`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.oracle_create.argtypes = [ctypes.POINTER(shhd.ShhdConfig), ctypes.POINTER(P)]
+    lib.oracle_create.restype = ctypes.c_int
+    lib.oracle_create_error.restype = ctypes.c_char_p
+    lib.oracle_destroy.argtypes = [P]
+    lib.oracle_last_error.argtypes = [P]
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_set_state.argtypes = [P, D, ctypes.c_size_t]
+    lib.oracle_get_state.argtypes = [P, D, ctypes.c_size_t]
+    lib.oracle_step.argtypes = [P, ctypes.c_int, ctypes.c_double]
+    lib.oracle_rhs_det.argtypes = [P, D, D, ctypes.c_double]
+    lib.oracle_rhs_sto.argtypes = [P, D]
+    lib.oracle_fft_c2c.argtypes = [ctypes.c_int, D, D, ctypes.c_int]
+    lib.oracle_fft2_r2c.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
+    lib.oracle_fft2_c2r.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
+    lib.oracle_set_threads.argtypes = [ctypes.c_int]
+    lib.oracle_set_threads.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    def __init__(self, cfg: "shhd.SolverConfig"):
+        self.lib = load()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        if self.lib.oracle_create(ctypes.byref(self._c), ctypes.byref(h)) != 0:
+            raise OracleError(self.lib.oracle_create_error().decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise OracleError(self.lib.oracle_last_error(self.h).decode())
+
+    def set_state(self, state):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        assert s.shape == self.cfg.state_shape()
+        f = s.view(np.float64).reshape(-1)
+        self._check(self.lib.oracle_set_state(self.h, _d(f), f.size))
+
+    def get_state(self):
+        out = np.zeros(self.cfg.state_shape(), dtype=np.complex128)
+        f = out.view(np.float64).reshape(-1)
+        self._check(self.lib.oracle_get_state(self.h, _d(f), f.size))
+        return out
+
+    def step(self, nsteps=1, t0=0.0):
+        self._check(self.lib.oracle_step(self.h, int(nsteps), float(t0)))
+
+    def rhs_det(self, state, t=0.0):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        out = np.zeros_like(s)
+        self._check(self.lib.oracle_rhs_det(self.h, _d(s.view(np.float64)), _d(out.view(np.float64)), float(t)))
+        return out
+
+    def rhs_sto(self):
+        out = np.zeros(self.cfg.state_shape(), dtype=np.complex128)
+        self._check(self.lib.oracle_rhs_sto(self.h, _d(out.view(np.float64))))
+        return out
+
+
+def fft_c2c(x, sign=-1):
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    assert lib.oracle_fft_c2c(x.size, _d(x.view(np.float64)), _d(out.view(np.float64)), sign) == 0
+    return out
+
+
+def fft2_r2c(phys):
+    lib = load()
+    phys = np.ascontiguousarray(phys, dtype=np.float64)
+    my, mx = phys.shape
+    out = np.zeros((my, mx // 2 + 1), dtype=np.complex128)
+    lib.oracle_fft2_r2c(mx, my, _d(phys), _d(out.view(np.float64)))
+    return out
+
+
+def fft2_c2r(spec, mx):
+    lib = load()
+    spec = np.ascontiguousarray(spec, dtype=np.complex128)
+    my = spec.shape[0]
+    out = np.zeros((my, mx), dtype=np.float64)
+    lib.oracle_fft2_c2r(mx, my, _d(spec.view(np.float64)), _d(out))
+    return out
+
+
+def set_threads(n):
+    return load().oracle_set_threads(int(n))

@@ -1,0 +1,127 @@
+// desc.h — the per-handle descriptor that the host lowers from shhd_config and
+// passes to every kernel by value (kernel-argument segment, read through the
+// scalar unit).
+#pragma once
+
+#include "lds_fft.h"
+
+namespace shhd {
+
+constexpr int kMaxOp = 8;                // SHHD_MAX_ORDER_PARAMETERS
+constexpr int kMaxFields = kMaxOp + 3;   // rho, psi_0..psi_{N-1}, jx, jy
+constexpr int kMaxIn = kMaxOp + 3;       // inverse-transformed fields
+constexpr int kMaxOut = 4 * kMaxOp + 5;  // forward-transformed products
+constexpr int kMaxGroups = kMaxOp + 1;   // one per psi_alpha + momentum
+constexpr int kMaxContrib = 5;
+constexpr int kMaxSources = 16;
+
+// Roles of the physical fields held per point in the row pass.
+enum InRole : int { ROLE_RHO = 0, ROLE_JX = 1, ROLE_JY = 2, ROLE_PSI0 = 3 };
+
+// Products formed in the row pass and forward transformed (the reference's
+// temporary_physical_fields contents, src/fcalculator.cc:189-494, 776-840).
+enum OutKind : int {
+    OUT_MU = 0,      // u_a psi_a^3                        (phi4 mu_phys)
+    OUT_PSIVX = 1,   // psi_a v_x
+    OUT_PSIVY = 2,   // psi_a v_y
+    OUT_GRADF = 3,   // sum_f -A_f v_dir(f)   (compressible force/gradient)
+    OUT_VX = 4,      // v_x                   (compressible viscous term)
+    OUT_VY = 5,
+    OUT_PXX = 6,     // j_x v_x               (momentum advection)
+    OUT_PXY = 7,     // j_x v_y
+    OUT_PYY = 8      // j_y v_y
+};
+
+enum GroupKind : int { GROUP_PSI = 0, GROUP_MOM = 1 };
+
+struct OutItem {
+    int kind;
+    int alpha;
+};
+
+struct Group {
+    int kind;        // GROUP_PSI or GROUP_MOM
+    int alpha;       // order parameter of a PSI group
+    int ncontrib;    // forward-transformed products feeding this group
+    int contrib[kMaxContrib];   // indices into the product list (U buffer)
+};
+
+struct Source {      // constant spectral source (sine force)
+    int field;       // state field index
+    int gx, cidx;    // compact mode
+    double re, im;
+};
+
+struct Desc {
+    // geometry: active grid Nax x Nay, compute grid Mx x My; the compact
+    // spectral column of gx holds Nay slots c with ky = c (c < Nay/2) or
+    // ky = c - Nay (c >= Nay/2); slot Nay/2 (ky = -Nay/2) is always zero.
+    int nax, nay, mx, my, hxa;
+    double lx, ly;
+    int N, nfields, mode;
+
+    // inverse column pass: state field of each inverse-transformed slot
+    int n_in;
+    int in_field[kMaxIn];
+    int in_role[kMaxIn];
+    double in_scale;            // 1/(Mx*My) of the backward transform
+
+    // row pass
+    int make_velocity;          // v = j / rho (compressible) or j / rho0
+    int velocity_compressible;
+    double inv_rho0;
+    int n_out;
+    OutItem out[kMaxOut];
+    double phi4_u[kMaxOp];
+    int n_grad;
+    int grad_component[8], grad_direction[8];
+    double grad_amplitude[8];
+
+    // forward column pass + stage update
+    int ngroups;
+    Group groups[kMaxGroups];
+    double dt, sqrt_dt;
+    double mobility[kMaxOp];
+    int psi_det_on[kMaxOp];     // psi_det does not return early (src/fcalculator.cc:95-97)
+    int psi_lin_on[kMaxOp];
+    double mu_k0[kMaxOp], mu_k2[kMaxOp], mu_k4[kMaxOp];
+    int mom_det_on;             // j_det does not return early (src/fcalculator.cc:131-133)
+    double pressure_coefficient;
+    int incompressible_viscous; // eta != 0 in incompressible mode
+    double eta, zeta;
+    double rho0;
+    int n_igrad;                // incompressible force/gradient terms (spectral)
+    int igrad_component[8], igrad_direction[8];
+    double igrad_amplitude[8];
+    int nsources;
+    Source sources[kMaxSources];
+
+    // noise (in-kernel Philox; src/fcalculator.cc:496-700)
+    int psi_noise_on[kMaxOp];
+    double psi_noise_pref[kMaxOp];   // 0.5 * P * M_a, sigma = sqrt(pref * k^2)
+    int j_noise_on;
+    double j_sigma, j_sqrt_eta, j_sqrt_zeta;
+    uint32_t seed_lo, seed_hi;
+
+    FftPlanDev plan_x, plan_y;
+};
+
+// Field pointers for one stage (static fields share storage across buffers).
+template <typename T>
+struct StagePtrs {
+    const cx<T>* cur[kMaxFields];
+    const cx<T>* base[kMaxFields];
+    cx<T>* next[kMaxFields];
+    // FP64 copies of the k = 0 mode (FP32 form only; null otherwise)
+    const cx<double>* cur_k0;
+    const cx<double>* base_k0;
+    cx<double>* next_k0;
+};
+
+struct StageParams {
+    double beta, wb, wc;
+    int use_b;              // SRK3: A + beta B; Euler: A only
+    uint32_t step_lo, step_hi;
+};
+
+}  // namespace shhd

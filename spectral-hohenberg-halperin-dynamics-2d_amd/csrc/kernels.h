@@ -1,0 +1,564 @@
+// kernels.h — the three kernels of one right-hand-side evaluation + stage
+// update, templated on the arithmetic type T (float | double), the workgroup
+// size NT and the values held per thread VPT.
+//
+//   K1 k_inverse_columns : compact spectra -> zero-padded inverse y transform
+//                          of every needed field; 1/(Mx*My) folded in.
+//                          (FCalculatorWorkspace::ensure_physical_fields,
+//                          src/fcalculator_workspace.cc:68-145, first half of
+//                          FourierTransform2D::backward_many)
+//   K2 k_rows            : per row y: inverse x transform (two real fields per
+//                          complex FFT), v = j/rho, the pointwise products
+//                          (mu_phys, psi v, j (x) v, v, force sources), then
+//                          forward x transform truncated to active kx.
+//                          (src/fcalculator.cc:189-274, 343-494, 776-840 and
+//                          the x halves of forward_many/backward_many)
+//   K3 k_forward_columns : forward y transform of the products, truncated to
+//                          active ky; spectral RHS (linear mu, pressure,
+//                          viscous, advection divergence, forces), in-kernel
+//                          noise, stage combination, mask, Hermitian kx = 0
+//                          line.  (src/fcalculator.cc:60-187, 496-700,
+//                          src/time_integrator.h:70-109 and the
+//                          calculate_stage loops of time_integrator_*.h)
+#pragma once
+
+#include "../../include/shhd_gpu.h"
+#include "desc.h"
+
+namespace shhd {
+
+// signed ky index of padded row gy (src/domain.cc:145-149)
+__device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2) ? gy : gy - my; }
+
+// ---------------------------------------------------------------------------
+// K1
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int VPT>
+__global__ __launch_bounds__(NT) void k_inverse_columns(Desc d, StagePtrs<T> P, cx<T>* __restrict__ tbuf,
+                                                        const cx<T>* __restrict__ twy, int C) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int slot = blockIdx.y;
+    const cx<T>* __restrict__ src = P.cur[d.in_field[slot]];
+    const int gx0 = blockIdx.x * C;
+    const int L = d.my;
+    const int ncol = C * L;
+    const int half = d.nay / 2;
+    const T scale = T(d.in_scale);
+    for (int idx = threadIdx.x; idx < ncol; idx += NT) {
+        const int c = idx / L;
+        const int y = idx - c * L;
+        const int gx = gx0 + c;
+        cx<T> v = czero<T>();
+        const int ky = signed_gy(y, L);
+        if (gx < d.hxa && ky < half && -ky < half) {
+            const int cidx = ky >= 0 ? ky : ky + d.nay;
+            v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+        }
+        lds[idx] = v;
+    }
+    __syncthreads();
+    lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, +1);
+    cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
+    for (int idx = threadIdx.x; idx < ncol; idx += NT) {
+        const int y = idx / C;
+        const int c = idx - y * C;
+        const int gx = gx0 + c;
+        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = lds[c * L + y];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2
+// ---------------------------------------------------------------------------
+template <typename T, int PPT, int MAXN>
+struct RowFields {
+    T rho[PPT], jx[PPT], jy[PPT], vx[PPT], vy[PPT];
+    T psi[MAXN > 0 ? MAXN : 1][PPT];
+
+    // i must be a compile-time constant after unrolling
+    __device__ __forceinline__ void store(int role, int i, T v) {
+        if (role == ROLE_RHO) rho[i] = v;
+        else if (role == ROLE_JX) jx[i] = v;
+        else if (role == ROLE_JY) jy[i] = v;
+        else {
+#pragma unroll
+            for (int a = 0; a < MAXN; ++a)
+                if (role == ROLE_PSI0 + a) psi[a][i] = v;
+        }
+    }
+    __device__ __forceinline__ T psi_of(int alpha, int i) const {
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < MAXN; ++a)
+            if (alpha == a) v = psi[a][i];
+        return v;
+    }
+    __device__ __forceinline__ T eval(const Desc& d, OutItem it, int i) const {
+        switch (it.kind) {
+            case OUT_MU: {
+                const T p = psi_of(it.alpha, i);
+                return T(d.phi4_u[it.alpha]) * p * p * p;   // src/model_free_energy_phi4.cc:54
+            }
+            case OUT_PSIVX: return psi_of(it.alpha, i) * vx[i];
+            case OUT_PSIVY: return psi_of(it.alpha, i) * vy[i];
+            case OUT_GRADF: {
+                T s = T(0);
+                for (int g = 0; g < d.n_grad; ++g) {
+                    if (d.grad_component[g] != it.alpha) continue;
+                    s += T(-d.grad_amplitude[g]) * (d.grad_direction[g] == 0 ? vx[i] : vy[i]);
+                }
+                return s;
+            }
+            case OUT_VX: return vx[i];
+            case OUT_VY: return vy[i];
+            case OUT_PXX: return jx[i] * vx[i];
+            case OUT_PXY: return jx[i] * vy[i];
+            default: return jy[i] * vy[i];
+        }
+    }
+};
+
+template <typename T, int NT, int PPT, int MAXN>
+__global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ tbuf, cx<T>* __restrict__ ubuf,
+                                             const cx<T>* __restrict__ twx, int R, int* __restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int L = d.mx;
+    const int my = d.my;
+    const int hxa = d.hxa;
+    const int y0 = blockIdx.x * R;
+    const int npts = R * L;
+    const size_t plane = static_cast<size_t>(my) * hxa;
+    const int tid = threadIdx.x;
+
+    RowFields<T, PPT, MAXN> F;
+
+    // ---- inverse x transforms, two fields per complex FFT -----------------
+    for (int s = 0; s < d.n_in; s += 2) {
+        const bool has_b = s + 1 < d.n_in;
+        const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
+        const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
+        for (int idx = tid; idx < npts; idx += NT) {
+            const int r = idx / L;
+            const int k = idx - r * L;
+            const int y = y0 + r;
+            cx<T> z = czero<T>();
+            if (y < my) {
+                const size_t row = static_cast<size_t>(y) * hxa;
+                if (k < hxa) {
+                    cx<T> a = ta[row + k];
+                    cx<T> b = has_b ? tb[row + k] : czero<T>();
+                    if (k == 0) { a.im = T(0); b.im = T(0); }
+                    z = mk(a.re - b.im, a.im + b.re);               // A + iB
+                } else if (k > L - hxa) {
+                    const int kk = L - k;
+                    const cx<T> a = ta[row + kk];
+                    const cx<T> b = has_b ? tb[row + kk] : czero<T>();
+                    z = mk(a.re + b.im, b.re - a.im);               // conj(A) + i conj(B)
+                }
+            }
+            lds[idx] = z;
+        }
+        __syncthreads();
+        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, +1);
+        const int ra = d.in_role[s];
+        const int rb = has_b ? d.in_role[s + 1] : -1;
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int idx = tid + NT * i;
+            if (idx < npts) {
+                const cx<T> z = lds[idx];
+                F.store(ra, i, z.re);
+                if (rb >= 0) F.store(rb, i, z.im);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- velocity (src/fcalculator_workspace.cc:122-133) ---------------------
+    if (d.make_velocity) {
+        const T r0 = T(d.rho0);
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int idx = tid + NT * i;
+            if (idx < npts) {
+                if (d.velocity_compressible) {
+                    const T r = F.rho[i];
+                    if (r == T(0) && y0 + idx / L < my) atomicOr(flag, 1);
+                    F.vx[i] = F.jx[i] / r;
+                    F.vy[i] = F.jy[i] / r;
+                } else {
+                    F.vx[i] = F.jx[i] / r0;
+                    F.vy[i] = F.jy[i] / r0;
+                }
+            }
+        }
+    }
+
+    // ---- products and forward x transforms, two per complex FFT -------------
+    for (int o = 0; o < d.n_out; o += 2) {
+        const bool has_b = o + 1 < d.n_out;
+        const OutItem ia = d.out[o];
+        const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int idx = tid + NT * i;
+            if (idx < npts) {
+                const T va = F.eval(d, ia, i);
+                const T vb = has_b ? F.eval(d, ib, i) : T(0);
+                lds[idx] = mk(va, vb);
+            }
+        }
+        __syncthreads();
+        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, -1);
+        cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
+        cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
+        const int nk = R * hxa;
+        for (int idx = tid; idx < nk; idx += NT) {
+            const int r = idx / hxa;
+            const int k = idx - r * hxa;
+            const int y = y0 + r;
+            if (y < my) {
+                const cx<T> zk = lds[r * L + k];
+                const cx<T> zm = lds[r * L + (k == 0 ? 0 : L - k)];
+                const size_t at = static_cast<size_t>(y) * hxa + k;
+                ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
+                if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// noise: one complex N(0,1) pair per Philox call
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cx<T> gauss_pair(const Desc& d, uint32_t mode_id, uint32_t stream, uint32_t draw,
+                                            const StageParams& sp) {
+    u32x4 ctr;
+    ctr.v[0] = mode_id;
+    ctr.v[1] = (stream << 4) | draw;
+    ctr.v[2] = sp.step_lo;
+    ctr.v[3] = sp.step_hi;
+    const u32x4 r = philox4x32_10(ctr, d.seed_lo, d.seed_hi);
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t a = (static_cast<uint64_t>(r.v[0]) << 21) | (r.v[1] >> 11);
+        const uint64_t b = (static_cast<uint64_t>(r.v[2]) << 21) | (r.v[3] >> 11);
+        const double u1 = (static_cast<double>(a) + 0.5) * 0x1.0p-53;
+        const double u2 = (static_cast<double>(b) + 0.5) * 0x1.0p-53;
+        const double rad = sqrt(-2.0 * log(u1));
+        double s, c;
+        sincospi(2.0 * u2, &s, &c);
+        return mk(rad * c, rad * s);
+    } else {
+        const float u1 = (static_cast<float>(r.v[0] >> 8) + 0.5f) * 0x1.0p-24f;
+        const float u2 = (static_cast<float>(r.v[1] >> 8) + 0.5f) * 0x1.0p-24f;
+        const float rad = sqrtf(-2.0f * logf(u1));
+        float s, c;
+        sincospif(2.0f * u2, &s, &c);
+        return mk(rad * c, rad * s);
+    }
+}
+
+// FP32 form: the k = 0 mode is advanced in FP64 from its FP64 copies; its
+// increment dt*det (noise vanishes at k = 0) comes from the FP32 pipeline and
+// the self-conjugate mode stays real (src/time_integrator.h:85-88).
+template <typename T>
+__device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<double>* cur_k0, cx<double>* next_k0,
+                                         cx<T>* dst, int f, cx<T> inc, const StageParams& sp) {
+    cx<double> nk;
+    nk.re = sp.wb * base_k0[f].re + sp.wc * (cur_k0[f].re + static_cast<double>(inc.re));
+    nk.im = 0.0;
+    next_k0[f] = nk;
+    *dst = mk(static_cast<T>(nk.re), T(0));
+}
+
+// ---------------------------------------------------------------------------
+// K3
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int VPT>
+__global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, StageParams sp,
+                                                        const cx<T>* __restrict__ ubuf,
+                                                        const cx<T>* __restrict__ twy, int C) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const Group& g = d.groups[blockIdx.y];
+    const int L = d.my;
+    const int nay = d.nay;
+    const int half = nay / 2;
+    const int hxa = d.hxa;
+    const int gx0 = blockIdx.x * C;
+    const int ncol = C * L;
+    const int nmodes = C * nay;
+    const size_t plane = static_cast<size_t>(L) * hxa;
+    const int tid = threadIdx.x;
+    const double two_pi = 2.0 * 3.14159265358979323846;
+
+    cx<T> acc0[VPT], acc1[VPT];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) { acc0[i] = czero<T>(); acc1[i] = czero<T>(); }
+
+    // ---- forward y transforms of the products feeding this group ------------
+    for (int ci = 0; ci < g.ncontrib; ++ci) {
+        const int u = g.contrib[ci];
+        const int kind = d.out[u].kind;
+        const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
+        for (int idx = tid; idx < ncol; idx += NT) {
+            const int y = idx / C;
+            const int c = idx - y * C;
+            const int gx = gx0 + c;
+            lds[c * L + y] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
+        }
+        __syncthreads();
+        lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int m = tid + NT * i;
+            if (m >= nmodes) continue;
+            const int c = m / nay;
+            const int cidx = m - c * nay;
+            const int gx = gx0 + c;
+            if (gx >= hxa || cidx == half) continue;
+            const int ky_i = cidx < half ? cidx : cidx - nay;
+            const int gy = ky_i >= 0 ? ky_i : ky_i + L;
+            const cx<T> v = lds[c * L + gy];
+            const T kx = T(two_pi * gx / d.lx);
+            const T ky = T(two_pi * ky_i / d.ly);
+            switch (kind) {
+                case OUT_MU: {   // src/fcalculator.cc:244-256
+                    const T mob = T(d.mobility[g.alpha]);
+                    const cx<T> fx = mul_mi((mob * kx) * v);
+                    const cx<T> fy = mul_mi((mob * ky) * v);
+                    acc0[i] = acc0[i] + mul_mi(kx * fx + ky * fy);
+                    break;
+                }
+                case OUT_PSIVX: acc0[i] = acc0[i] + mul_mi(kx * v); break;
+                case OUT_PSIVY: acc0[i] = acc0[i] + mul_mi(ky * v); break;
+                case OUT_GRADF: acc0[i] = acc0[i] + v; break;
+                case OUT_VX: {   // src/fcalculator.cc:378-392 with vy = 0
+                    const T eta = T(d.eta), zeta = T(d.zeta);
+                    const cx<T> pxx = mul_mi((T(2) * eta * kx - (eta - zeta) * kx) * v);
+                    const cx<T> pxy = mul_mi((eta * ky) * v);
+                    const cx<T> pyy = mul_mi((-(eta - zeta) * kx) * v);
+                    acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
+                    acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
+                    break;
+                }
+                case OUT_VY: {   // same with vx = 0
+                    const T eta = T(d.eta), zeta = T(d.zeta);
+                    const cx<T> pxx = mul_mi((-(eta - zeta) * ky) * v);
+                    const cx<T> pxy = mul_mi((eta * kx) * v);
+                    const cx<T> pyy = mul_mi((T(2) * eta * ky - (eta - zeta) * ky) * v);
+                    acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
+                    acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
+                    break;
+                }
+                case OUT_PXX: acc0[i] = acc0[i] + mul_mi(kx * v); break;
+                case OUT_PXY:
+                    acc0[i] = acc0[i] + mul_mi(ky * v);
+                    acc1[i] = acc1[i] + mul_mi(kx * v);
+                    break;
+                default: acc1[i] = acc1[i] + mul_mi(ky * v); break;   // OUT_PYY
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- park the accumulated product terms in LDS (free after the last
+    // barrier) so the stage update below can walk the modes in a plain loop
+    cx<T>* acc_x = lds;
+    cx<T>* acc_y = lds + nmodes;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+        const int m = tid + NT * i;
+        if (m < nmodes) {
+            acc_x[m] = acc0[i];
+            acc_y[m] = acc1[i];
+        }
+    }
+    __syncthreads();
+
+    // ---- spectral terms, noise, stage update --------------------------------
+    const bool psi_group = g.kind == GROUP_PSI;
+    const int a = g.alpha;
+    const bool compressible = d.mode == SHHD_MODE_COMPRESSIBLE;
+    const int fx_field = d.N + 1, fy_field = d.N + 2;
+    const T dt = T(d.dt), sdt = T(d.sqrt_dt);
+    const T wb = T(sp.wb), wc = T(sp.wc), beta = T(sp.beta);
+    constexpr bool kFp32 = sizeof(T) == 4;
+
+    for (int m = tid; m < nmodes; m += NT) {
+        const int c = m / nay;
+        const int cidx = m - c * nay;
+        const int gx = gx0 + c;
+        if (gx >= hxa) continue;
+        const size_t at = static_cast<size_t>(gx) * nay + cidx;
+        if (cidx == half) {   // the always-zero ky = -Nay/2 slot
+            if (psi_group) {
+                P.next[1 + a][at] = czero<T>();
+            } else {
+                if (compressible) P.next[0][at] = czero<T>();
+                P.next[fx_field][at] = czero<T>();
+                P.next[fy_field][at] = czero<T>();
+            }
+            continue;
+        }
+        const int ky_i = cidx < half ? cidx : cidx - nay;
+        const T kx = T(two_pi * gx / d.lx);
+        const T ky = T(two_pi * ky_i / d.ly);
+        const T k2 = kx * kx + ky * ky;
+        const bool is_k0 = kFp32 && gx == 0 && cidx == 0 && P.next_k0 != nullptr;
+        // canonical mode of the noise draw: the kx = 0 line draws the ky > 0
+        // member of each pair and conjugates it onto ky < 0
+        // (src/fcalculator.cc:525-566)
+        const bool conj_draw = (gx == 0 && ky_i < 0);
+        const uint32_t mode_id = static_cast<uint32_t>(gx * nay + (conj_draw ? -ky_i : cidx));
+
+        if (psi_group) {
+            const int f = 1 + a;
+            const cx<T> cur = P.cur[f][at];
+            cx<T> det = acc_x[m];
+            if (d.psi_det_on[a]) {
+                if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
+                    const T mob = T(d.mobility[a]);
+                    const cx<T> mu = (T(d.mu_k0[a]) + T(d.mu_k2[a]) * k2 + T(d.mu_k4[a]) * k2 * k2) * cur;
+                    const cx<T> fx = mul_mi((mob * kx) * mu);
+                    const cx<T> fy = mul_mi((mob * ky) * mu);
+                    det = det + mul_mi(kx * fx + ky * fy);
+                }
+                for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
+                    if (d.igrad_component[q] != a) continue;
+                    const cx<T> j = P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field][at];
+                    const cx<T> t = T(-d.igrad_amplitude[q]) * j;
+                    det = det + mk(t.re / T(d.rho0), t.im / T(d.rho0));
+                }
+                for (int q = 0; q < d.nsources; ++q) {
+                    if (d.sources[q].field == f && d.sources[q].gx == gx && d.sources[q].cidx == cidx)
+                        det = det + mk(T(d.sources[q].re), T(d.sources[q].im));
+                }
+            }
+            cx<T> sto = czero<T>();
+            if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
+                const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
+                const cx<T> xa = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 0u, sp);
+                cx<T> na = sigma * xa;
+                if (sp.use_b) {
+                    const cx<T> xb = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 1u, sp);
+                    na = na + beta * (sigma * xb);
+                }
+                sto = conj_draw ? conj(na) : na;
+            }
+            if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
+            else P.next[f][at] = wb * P.base[f][at] + wc * (cur + dt * det + sdt * sto);
+        } else {
+            const cx<T> jx = P.cur[fx_field][at];
+            const cx<T> jy = P.cur[fy_field][at];
+            cx<T> dx = acc_x[m], dy = acc_y[m];
+            if (d.mom_det_on) {
+                if (d.pressure_coefficient != 0.0) {   // src/fcalculator.cc:276-300
+                    const cx<T> p = T(d.pressure_coefficient) * P.cur[0][at];
+                    dx = dx + mul_mi(kx * p);
+                    dy = dy + mul_mi(ky * p);
+                }
+                if (d.incompressible_viscous) {   // src/fcalculator.cc:302-341
+                    const T eta = T(d.eta);
+                    const T r0 = T(d.rho0);
+                    const cx<T> vx = mk(jx.re / r0, jx.im / r0);
+                    const cx<T> vy = mk(jy.re / r0, jy.im / r0);
+                    const cx<T> pxx = mul_mi(eta * ((T(2) * kx) * vx));
+                    const cx<T> pxy = mul_mi(eta * (ky * vx + kx * vy));
+                    const cx<T> pyy = mul_mi(eta * ((T(2) * ky) * vy));
+                    dx = dx + mul_mi(kx * pxx + ky * pxy);
+                    dy = dy + mul_mi(kx * pxy + ky * pyy);
+                }
+                for (int q = 0; q < d.nsources; ++q) {
+                    if (d.sources[q].gx != gx || d.sources[q].cidx != cidx) continue;
+                    const cx<T> sv = mk(T(d.sources[q].re), T(d.sources[q].im));
+                    if (d.sources[q].field == fx_field) dx = dx + sv;
+                    if (d.sources[q].field == fy_field) dy = dy + sv;
+                }
+            }
+            cx<T> sx = czero<T>(), sy = czero<T>();
+            if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
+                const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
+                const T kxd = conj_draw ? T(0) : kx;
+                const T kyd = conj_draw ? -ky : ky;
+                const int ndraw = sp.use_b ? 2 : 1;
+                for (int ab = 0; ab < ndraw; ++ab) {
+                    const cx<T> x1 = gauss_pair<T>(d, mode_id, 8u, static_cast<uint32_t>(ab), sp);
+                    const cx<T> x2 = gauss_pair<T>(d, mode_id, 9u, static_cast<uint32_t>(ab), sp);
+                    const cx<T> x3 = gauss_pair<T>(d, mode_id, 10u, static_cast<uint32_t>(ab), sp);
+                    const cx<T> pxx = -ss * (sz * x1 + se * x2);
+                    const cx<T> pyy = -ss * (sz * x1 - se * x2);
+                    const cx<T> pxy = (-ss * se) * x3;
+                    cx<T> vx = mul_mi(kxd * pxx + kyd * pxy);
+                    cx<T> vy = mul_mi(kxd * pxy + kyd * pyy);
+                    if (conj_draw) { vx = conj(vx); vy = conj(vy); }
+                    if (ab == 0) { sx = vx; sy = vy; }
+                    else { sx = sx + beta * vx; sy = sy + beta * vy; }
+                }
+            }
+            if (compressible) {
+                const cx<T> drho = mul_mi(kx * jx + ky * jy);   // src/fcalculator.cc:60-84
+                if (is_k0) {
+                    k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[0] + at, 0, dt * drho, sp);
+                    k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
+                    k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
+                } else {
+                    P.next[0][at] = wb * P.base[0][at] + wc * (P.cur[0][at] + dt * drho);
+                    P.next[fx_field][at] = wb * P.base[fx_field][at] + wc * (jx + dt * dx + sdt * sx);
+                    P.next[fy_field][at] = wb * P.base[fy_field][at] + wc * (jy + dt * dy + sdt * sy);
+                }
+            } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
+                cx<T> ix = dt * dx + sdt * sx;
+                cx<T> iy = dt * dy + sdt * sy;
+                if (k2 != T(0)) {
+                    const cx<T> tr = (-ky) * ix + kx * iy;
+                    const cx<T> px = (-ky) * tr;
+                    const cx<T> py = kx * tr;
+                    ix = mk(px.re / k2, px.im / k2);
+                    iy = mk(py.re / k2, py.im / k2);
+                }
+                if (is_k0) {
+                    k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
+                    k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
+                } else {
+                    P.next[fx_field][at] = wb * P.base[fx_field][at] + wc * (jx + ix);
+                    P.next[fy_field][at] = wb * P.base[fy_field][at] + wc * (jy + iy);
+                }
+            }
+        }
+    }
+
+    // ---- Hermitian projection of the kx = 0 line (src/time_integrator.h:70-103)
+    // The block owning gx = 0 re-reads its own writes after a workgroup barrier
+    // (which orders the global stores within the workgroup); each (ky, -ky)
+    // pair is handled by one thread.
+    if (gx0 == 0) {
+        __threadfence_block();
+        __syncthreads();
+        int nf = 0;
+        int wf0 = 0, wf1 = 0, wf2 = 0;
+        if (psi_group) { wf0 = 1 + a; nf = 1; }
+        else if (compressible) { wf0 = 0; wf1 = fx_field; wf2 = fy_field; nf = 3; }
+        else { wf0 = fx_field; wf1 = fy_field; nf = 2; }
+        for (int q = 0; q < nf; ++q) {
+            cx<T>* col = P.next[q == 0 ? wf0 : (q == 1 ? wf1 : wf2)];   // column gx = 0 starts at index 0
+            for (int m = tid; m < half; m += NT) {
+                if (m == 0) {
+                    col[0].im = T(0);
+                } else {
+                    const cx<T> av = col[m];
+                    const cx<T> bv = col[nay - m];
+                    const cx<T> p = T(0.5) * (av + conj(bv));
+                    col[m] = p;
+                    col[nay - m] = conj(p);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace shhd

@@ -1,0 +1,138 @@
+// lds_fft.h — batched complex FFT of sequences held in LDS, executed by one
+// workgroup of NT threads.
+//
+// Stockham autosort formulation with radix-2/3/4/8 passes: pass p with radix R
+// and current sub-transform length ns turns butterfly j of a sequence of
+// length L (j < L/R, k = j mod ns) into
+//     v_q = x[j + q L/R] * w_L^{sign q k L/(ns R)},   q < R
+//     y[(j/ns) ns R + k + p ns] = sum_q v_q w_R^{sign p q},   p < R
+// so the output is in natural order after the last pass.  Each pass reads all
+// of a thread's butterfly inputs into registers, synchronises, then writes the
+// outputs back in place, so one LDS buffer of nseq*L values suffices.
+//
+// Twiddles come from a table tw[j] = exp(-2 pi i j / L), j < L, computed in
+// FP64 on the host (and rounded for the FP32 form).
+#pragma once
+
+#include "cx.h"
+
+namespace shhd {
+
+constexpr int kMaxPasses = 16;
+
+struct FftPlanDev {
+    int npass;
+    int radix[kMaxPasses];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void dft_small(cx<T> (&v)[R], int sign) {
+    if constexpr (R == 2) {
+        const cx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (R == 3) {
+        const T s3 = T(0.86602540378443864676);   // sin(2 pi / 3)
+        const cx<T> t1 = v[1] + v[2];
+        const cx<T> t2 = v[0] - T(0.5) * t1;
+        const cx<T> d = v[1] - v[2];
+        // sign * i * s3 * d
+        const cx<T> r = (sign > 0) ? mul_i(s3 * d) : mul_mi(s3 * d);
+        v[0] = v[0] + t1;
+        v[1] = t2 + r;
+        v[2] = t2 - r;
+    } else if constexpr (R == 4) {
+        const cx<T> a0 = v[0] + v[2], a1 = v[0] - v[2];
+        const cx<T> b0 = v[1] + v[3], b1 = v[1] - v[3];
+        const cx<T> r = (sign > 0) ? mul_i(b1) : mul_mi(b1);
+        v[0] = a0 + b0;
+        v[2] = a0 - b0;
+        v[1] = a1 + r;
+        v[3] = a1 - r;
+    } else if constexpr (R == 8) {
+        cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<T, 4>(e, sign);
+        dft_small<T, 4>(o, sign);
+        const T h = T(0.70710678118654752440);
+        // w8^p o_p with w8 = exp(sign i pi / 4)
+        const cx<T> o1 = (sign > 0) ? mk(h * (o[1].re - o[1].im), h * (o[1].re + o[1].im))
+                                    : mk(h * (o[1].re + o[1].im), h * (o[1].im - o[1].re));
+        const cx<T> o2 = (sign > 0) ? mul_i(o[2]) : mul_mi(o[2]);
+        const cx<T> o3 = (sign > 0) ? mk(-h * (o[3].re + o[3].im), h * (o[3].re - o[3].im))
+                                    : mk(h * (o[3].im - o[3].re), -h * (o[3].re + o[3].im));
+        v[0] = e[0] + o[0];
+        v[4] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[5] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[6] = e[2] - o2;
+        v[3] = e[3] + o3;
+        v[7] = e[3] - o3;
+    }
+}
+
+template <typename T, int NT, int VPT, int R>
+__device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, int ns, const cx<T>* __restrict__ tw,
+                                         int sign) {
+    constexpr int B = (VPT + R - 1) / R;
+    const int m = L / R;
+    const int nb = nseq * m;
+    const int tstep = L / (ns * R);
+    cx<T> v[B][R];
+    int jj[B], base[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int id = static_cast<int>(threadIdx.x) + NT * b;
+        jj[b] = 0;
+        base[b] = 0;
+        if (id < nb) {
+            const int s = id / m;
+            const int j = id - s * m;
+            jj[b] = j;
+            base[b] = s * L;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b][q] = buf[s * L + j + q * m];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int id = static_cast<int>(threadIdx.x) + NT * b;
+        if (id < nb) {
+            const int j = jj[b];
+            const int k = j % ns;
+            if (k != 0) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    cx<T> w = tw[q * k * tstep];
+                    if (sign > 0) w.im = -w.im;
+                    v[b][q] = v[b][q] * w;
+                }
+            }
+            dft_small<T, R>(v[b], sign);
+            const int out = base[b] + (j / ns) * ns * R + k;
+#pragma unroll
+            for (int p = 0; p < R; ++p) buf[out + p * ns] = v[b][p];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place FFT of nseq sequences of length L stored back to back in buf.
+// Requires nseq * L <= NT * VPT and the plan's radices to multiply to L.
+template <typename T, int NT, int VPT>
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
+                                        const cx<T>* __restrict__ tw, int sign) {
+    int ns = 1;
+    for (int p = 0; p < plan.npass; ++p) {
+        const int r = plan.radix[p];
+        if (r == 8) fft_pass<T, NT, VPT, 8>(buf, L, nseq, ns, tw, sign);
+        else if (r == 4) fft_pass<T, NT, VPT, 4>(buf, L, nseq, ns, tw, sign);
+        else if (r == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ns, tw, sign);
+        else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ns, tw, sign);
+        ns *= r;
+    }
+}
+
+}  // namespace shhd

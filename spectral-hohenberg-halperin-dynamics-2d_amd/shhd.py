@@ -1,0 +1,336 @@
+"""ctypes binding of libshhd_gpu.so (declared in include/shhd_gpu.h).
+
+`GpuFluidSolver` mirrors the reference's `FluidSolver` (src/fluid_solver.h:23-108):
+it is built from the lowered configuration, and `step(state, t)` advances a
+reference-layout spectral state by one time step.  State arrays are complex128
+numpy arrays of shape (N+3, My, Mx//2+1) with fields ordered
+[rho, psi_0..psi_{N-1}, jx, jy] (src/state.cc:8-38).
+
+The library must have been built (`make` in this directory); importing this
+module on a machine without it raises immediately — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshhd_gpu.so")
+
+MAX_OP = 8
+MAX_FORCES = 8
+
+MODE_COMPRESSIBLE, MODE_INCOMPRESSIBLE, MODE_QUIESCENT = 0, 1, 2
+INTEGRATOR_EULER, INTEGRATOR_SRK3 = 0, 1
+FREE_ENERGY_NULL, FREE_ENERGY_QUADRATIC, FREE_ENERGY_PHI4 = 0, 1, 2
+
+STATUS = {0: "SHHD_OK", 1: "SHHD_ERR_INVALID", 2: "SHHD_ERR_UNSUPPORTED", 3: "SHHD_ERR_HIP",
+          4: "SHHD_ERR_ZERO_DENSITY", 5: "SHHD_ERR_RUNTIME"}
+
+
+class ShhdSineForce(ctypes.Structure):
+    _fields_ = [("momentum", ctypes.c_int), ("component", ctypes.c_int), ("axis", ctypes.c_int),
+                ("nk", ctypes.c_int), ("amplitude", ctypes.c_double)]
+
+
+class ShhdGradientForce(ctypes.Structure):
+    _fields_ = [("component", ctypes.c_int), ("direction", ctypes.c_int), ("amplitude", ctypes.c_double)]
+
+
+class ShhdConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx_active", ctypes.c_int), ("ny_active", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("num_order_parameters", ctypes.c_int),
+        ("mode", ctypes.c_int), ("integrator", ctypes.c_int),
+        ("dt", ctypes.c_double), ("precision", ctypes.c_int),
+        ("pressure_coefficient", ctypes.c_double),
+        ("eta", ctypes.c_double), ("zeta", ctypes.c_double),
+        ("mobility", ctypes.c_double * MAX_OP),
+        ("free_energy_kind", ctypes.c_int),
+        ("mu_k0", ctypes.c_double * MAX_OP), ("mu_k2", ctypes.c_double * MAX_OP),
+        ("mu_k4", ctypes.c_double * MAX_OP), ("phi4_u", ctypes.c_double * MAX_OP),
+        ("momentum_advection", ctypes.c_int), ("order_parameter_advection", ctypes.c_int),
+        ("noise_momentum", ctypes.c_int), ("noise_order_parameter", ctypes.c_int),
+        ("kBT", ctypes.c_double), ("order_parameter_noise_chi", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("num_sine_forces", ctypes.c_int), ("sine_forces", ShhdSineForce * MAX_FORCES),
+        ("num_gradient_forces", ctypes.c_int), ("gradient_forces", ShhdGradientForce * MAX_FORCES),
+    ]
+
+
+class ShhdInfo(ctypes.Structure):
+    _fields_ = [("num_fields", ctypes.c_int), ("num_inverse_fields", ctypes.c_int),
+                ("num_forward_fields", ctypes.c_int), ("num_updated_fields", ctypes.c_int),
+                ("stages_per_step", ctypes.c_int), ("state_doubles", ctypes.c_size_t),
+                ("device_bytes", ctypes.c_size_t)]
+
+
+@dataclass
+class SolverConfig:
+    """Plain-Python view of shhd_config with the reference's defaults
+    (src/simulationinfo.h:35-124: grid 128^2, L = 2 pi, dt 0.005, srk3/compressible,
+    noise seed 12345 and kBT 1)."""
+    nx_active: int = 128
+    ny_active: int = 128
+    dealias: str = "none"           # none | 3/2 | 2 (GridConfig::compute_grid_size)
+    lx: float = 2.0 * math.pi
+    ly: float = 2.0 * math.pi
+    num_order_parameters: int = 0
+    mode: int = MODE_COMPRESSIBLE
+    integrator: int = INTEGRATOR_SRK3
+    dt: float = 0.005
+    precision: int = 64
+    pressure_coefficient: float = 0.0
+    eta: float = 0.0
+    zeta: float = 0.0
+    mobility: Sequence[float] = ()
+    free_energy_kind: int = FREE_ENERGY_NULL
+    mu_k0: Sequence[float] = ()
+    mu_k2: Sequence[float] = ()
+    mu_k4: Sequence[float] = ()
+    phi4_u: Sequence[float] = ()
+    momentum_advection: bool = False
+    order_parameter_advection: bool = False
+    noise_momentum: bool = False
+    noise_order_parameter: bool = False
+    kBT: float = 1.0
+    order_parameter_noise_chi: float = 1.0
+    seed: int = 12345
+    sine_forces: List[dict] = field(default_factory=list)
+    gradient_forces: List[dict] = field(default_factory=list)
+
+    @property
+    def nx(self) -> int:
+        return compute_grid_size(self.nx_active, self.dealias)
+
+    @property
+    def ny(self) -> int:
+        return compute_grid_size(self.ny_active, self.dealias)
+
+    @property
+    def num_fields(self) -> int:
+        return self.num_order_parameters + 3
+
+    def state_shape(self):
+        return (self.num_fields, self.ny, self.nx // 2 + 1)
+
+    def to_c(self) -> ShhdConfig:
+        c = ShhdConfig()
+        c.nx_active, c.ny_active = self.nx_active, self.ny_active
+        c.nx, c.ny = self.nx, self.ny
+        c.lx, c.ly = self.lx, self.ly
+        c.num_order_parameters = self.num_order_parameters
+        c.mode, c.integrator = self.mode, self.integrator
+        c.dt, c.precision = self.dt, self.precision
+        c.pressure_coefficient, c.eta, c.zeta = self.pressure_coefficient, self.eta, self.zeta
+        n = self.num_order_parameters
+        for name in ("mobility", "mu_k0", "mu_k2", "mu_k4", "phi4_u"):
+            vals = list(getattr(self, name))
+            arr = getattr(c, name)
+            for a in range(MAX_OP):
+                arr[a] = float(vals[a]) if a < len(vals) else 0.0
+            if len(vals) > n and any(v != 0 for v in vals[n:]):
+                raise ValueError(f"{name} has more entries than order parameters")
+        c.free_energy_kind = self.free_energy_kind
+        c.momentum_advection = int(bool(self.momentum_advection))
+        c.order_parameter_advection = int(bool(self.order_parameter_advection))
+        c.noise_momentum = int(bool(self.noise_momentum))
+        c.noise_order_parameter = int(bool(self.noise_order_parameter))
+        c.kBT, c.order_parameter_noise_chi = self.kBT, self.order_parameter_noise_chi
+        c.seed = int(self.seed)
+        c.num_sine_forces = len(self.sine_forces)
+        for i, f in enumerate(self.sine_forces):
+            c.sine_forces[i] = ShhdSineForce(int(f["momentum"]), int(f["component"]), int(f["axis"]),
+                                             int(f["nk"]), float(f["amplitude"]))
+        c.num_gradient_forces = len(self.gradient_forces)
+        for i, f in enumerate(self.gradient_forces):
+            c.gradient_forces[i] = ShhdGradientForce(int(f["component"]), int(f["direction"]),
+                                                     float(f["amplitude"]))
+        return c
+
+
+def compute_grid_size(active: int, rule: str) -> int:
+    """GridConfig::compute_grid_size (src/simulationinfo.h:44-55)."""
+    if active <= 0:
+        raise ValueError("grid size must be positive.")
+    if rule in ("none", "off"):
+        return active
+    if rule in ("3/2", "three_halves"):
+        if active % 2:
+            raise ValueError("3/2 dealiasing requires even active grid size.")
+        return 3 * active // 2
+    if rule in ("2", "two"):
+        return 2 * active
+    raise ValueError("unknown dealias rule: " + rule)
+
+
+# ---------------------------------------------------------------------------
+# layout helpers (src/domain.cc:139-149, src/spectral_mask.cc:45-52)
+# ---------------------------------------------------------------------------
+def wavenumbers(cfg: SolverConfig):
+    gx = np.arange(cfg.nx // 2 + 1)
+    gy = np.arange(cfg.ny)
+    kx = 2.0 * math.pi * gx / cfg.lx
+    sy = np.where(gy <= cfg.ny // 2, gy, gy - cfg.ny)
+    ky = 2.0 * math.pi * sy / cfg.ly
+    return kx[None, :], ky[:, None], sy
+
+
+def active_mask(cfg: SolverConfig) -> np.ndarray:
+    gx = np.arange(cfg.nx // 2 + 1)[None, :]
+    gy = np.arange(cfg.ny)[:, None]
+    sy = np.where(gy <= cfg.ny // 2, gy, gy - cfg.ny)
+    return (gx < cfg.nx_active // 2) & (np.abs(sy) < cfg.ny_active // 2)
+
+
+def hermitian_project(state: np.ndarray, cfg: SolverConfig) -> np.ndarray:
+    """Mask + kx = 0 Hermitian projection (src/time_integrator.h:70-109)."""
+    s = np.where(active_mask(cfg)[None], state, 0)
+    ny = cfg.ny
+    out = s.copy()
+    for gy in range(ny):
+        cg = (ny - gy) % ny
+        if not active_mask(cfg)[gy, 0]:
+            continue
+        if cg == gy:
+            out[:, gy, 0] = out[:, gy, 0].real
+        elif gy < cg:
+            p = 0.5 * (out[:, gy, 0] + np.conj(out[:, cg, 0]))
+            out[:, gy, 0] = p
+            out[:, cg, 0] = np.conj(p)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# library
+# ---------------------------------------------------------------------------
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libshhd_gpu.so is not built ({path}); run `make` in {HERE}")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    lib.shhd_gpu_create.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.POINTER(P)]
+    lib.shhd_gpu_create.restype = ctypes.c_int
+    lib.shhd_gpu_set_state.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    lib.shhd_gpu_set_state.restype = ctypes.c_int
+    lib.shhd_gpu_get_state.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    lib.shhd_gpu_get_state.restype = ctypes.c_int
+    lib.shhd_gpu_step.argtypes = [P, ctypes.c_int, ctypes.c_double]
+    lib.shhd_gpu_step.restype = ctypes.c_int
+    lib.shhd_gpu_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.shhd_gpu_set_flux_request.restype = ctypes.c_int
+    lib.shhd_gpu_get_flux.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    lib.shhd_gpu_get_flux.restype = ctypes.c_int
+    lib.shhd_gpu_destroy.argtypes = [P]
+    lib.shhd_gpu_destroy.restype = None
+    lib.shhd_gpu_last_error.argtypes = [P]
+    lib.shhd_gpu_last_error.restype = ctypes.c_char_p
+    lib.shhd_gpu_get_info.argtypes = [P, ctypes.POINTER(ShhdInfo)]
+    lib.shhd_gpu_get_info.restype = ctypes.c_int
+    lib.shhd_gpu_set_kernel_timing.argtypes = [P, ctypes.c_int]
+    lib.shhd_gpu_set_kernel_timing.restype = ctypes.c_int
+    lib.shhd_gpu_get_kernel_timing.argtypes = [P, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    lib.shhd_gpu_get_kernel_timing.restype = ctypes.c_int
+    lib.shhd_gpu_get_kernel_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.shhd_gpu_get_kernel_bytes.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+class ShhdError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class GpuFluidSolver:
+    """Drop-in for the reference FluidSolver's step path on one HIP device."""
+
+    KERNEL_CLASSES = ("inverse_columns", "rows", "forward_columns_stage")
+
+    def __init__(self, cfg: SolverConfig):
+        self.lib = load_library()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        st = self.lib.shhd_gpu_create(ctypes.byref(self._c), ctypes.byref(h))
+        if st != 0:
+            raise ShhdError(st, self.lib.shhd_gpu_last_error(None).decode())
+        self.h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise ShhdError(st, self.lib.shhd_gpu_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.shhd_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, state: np.ndarray):
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        if s.shape != self.cfg.state_shape():
+            raise ValueError(f"state shape {s.shape} != {self.cfg.state_shape()}")
+        flat = s.view(np.float64).reshape(-1)
+        self._check(self.lib.shhd_gpu_set_state(self.h, _dptr(flat), flat.size))
+
+    def get_state(self) -> np.ndarray:
+        out = np.zeros(self.cfg.state_shape(), dtype=np.complex128)
+        flat = out.view(np.float64).reshape(-1)
+        self._check(self.lib.shhd_gpu_get_state(self.h, _dptr(flat), flat.size))
+        return out
+
+    def step(self, nsteps: int = 1, t0: float = 0.0):
+        self._check(self.lib.shhd_gpu_step(self.h, int(nsteps), float(t0)))
+
+    def set_flux_request(self, density=False, order_parameter=False, momentum=False):
+        self._check(self.lib.shhd_gpu_set_flux_request(self.h, int(density), int(order_parameter), int(momentum)))
+
+    def info(self) -> ShhdInfo:
+        i = ShhdInfo()
+        self._check(self.lib.shhd_gpu_get_info(self.h, ctypes.byref(i)))
+        return i
+
+    def set_kernel_timing(self, enable: bool):
+        self._check(self.lib.shhd_gpu_set_kernel_timing(self.h, int(enable)))
+
+    def kernel_timing(self):
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_longlong * 3)()
+        self._check(self.lib.shhd_gpu_get_kernel_timing(self.h, ms, n, 3))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    def kernel_bytes(self):
+        b = (ctypes.c_double * 3)()
+        self._check(self.lib.shhd_gpu_get_kernel_bytes(self.h, b, 3))
+        return {k: b[i] for i, k in enumerate(self.KERNEL_CLASSES)}
