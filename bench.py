@@ -148,7 +148,17 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form side measurement")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a device-to-device copy of MB megabytes (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(args.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 8, "ok": bool(y[0].item() == 1.0)}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration of this path fits one MI355X and runs are "
               "single-GPU only (see the sharding design in DESIGN.md).", file=sys.stderr)

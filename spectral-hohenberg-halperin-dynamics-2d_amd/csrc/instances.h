@@ -1,0 +1,32 @@
+// instances.h — the (lanes, values-per-lane) shapes the kernels are built for.
+// Each instance is compiled once, in one of the inst_*.hip translation units;
+// the host code only refers to them.
+#pragma once
+
+#include "kernels.h"
+
+// X(NT, V): NT lanes per workgroup, V values per lane (6 for 3*2^a sides, 8 for 2^a)
+#define SHHD_FOR_NT_V(X) X(64, 6) X(64, 8) X(256, 6) X(256, 8) X(512, 6) X(512, 8) X(1024, 6) X(1024, 8)
+
+namespace shhd {
+
+#define SHHD_DECL_K1(T, nt, v) \
+    extern template __global__ void k_inverse_columns<T, nt, v>(Desc, StagePtrs<T>, cx<T>*, const cx<T>*, int);
+#define SHHD_DECL_K3(T, nt, v, a)                                                                              \
+    extern template __global__ void k_forward_columns<T, nt, v, a>(Desc, StagePtrs<T>, StageParams, const cx<T>*, \
+                                                                   const cx<T>*, int);
+#define SHHD_DECL_K2(T, nt, v, mn) \
+    extern template __global__ void k_rows<T, nt, v, mn>(Desc, const cx<T>*, cx<T>*, const cx<T>*, int, int*);
+
+#define SHHD_DECL_ALL(T, nt, v)                                                              \
+    SHHD_DECL_K1(T, nt, v) SHHD_DECL_K3(T, nt, v, 4) SHHD_DECL_K3(T, nt, v, v)                \
+    SHHD_DECL_K2(T, nt, v, 0) SHHD_DECL_K2(T, nt, v, 1) SHHD_DECL_K2(T, nt, v, 2)            \
+    SHHD_DECL_K2(T, nt, v, 4) SHHD_DECL_K2(T, nt, v, 8)
+#define SHHD_DECL_F64(nt, v) SHHD_DECL_ALL(double, nt, v)
+#define SHHD_DECL_F32(nt, v) SHHD_DECL_ALL(float, nt, v)
+#ifndef SHHD_DEFINE_INSTANCES
+SHHD_FOR_NT_V(SHHD_DECL_F64)
+SHHD_FOR_NT_V(SHHD_DECL_F32)
+#endif
+
+}  // namespace shhd

@@ -278,7 +278,7 @@ __device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<dou
 // ---------------------------------------------------------------------------
 // K3
 // ---------------------------------------------------------------------------
-template <typename T, int NT, int VPT>
+template <typename T, int NT, int VPT, int ACC>
 __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, StageParams sp,
                                                         const cx<T>* __restrict__ ubuf,
                                                         const cx<T>* __restrict__ twy, int C) {
@@ -296,9 +296,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, 
     const int tid = threadIdx.x;
     const double two_pi = 2.0 * 3.14159265358979323846;
 
-    cx<T> acc0[VPT], acc1[VPT];
+    // ACC >= ceil(C * Nay / NT) accumulator slots per lane (host-checked)
+    cx<T> acc0[ACC], acc1[ACC];
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) { acc0[i] = czero<T>(); acc1[i] = czero<T>(); }
+    for (int i = 0; i < ACC; ++i) { acc0[i] = czero<T>(); acc1[i] = czero<T>(); }
 
     // ---- forward y transforms of the products feeding this group ------------
     for (int ci = 0; ci < g.ncontrib; ++ci) {
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, 
         __syncthreads();
         lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
 #pragma unroll
-        for (int i = 0; i < VPT; ++i) {
+        for (int i = 0; i < ACC; ++i) {
             const int m = tid + NT * i;
             if (m >= nmodes) continue;
             const int c = m / nay;
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, 
     cx<T>* acc_x = lds;
     cx<T>* acc_y = lds + nmodes;
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) {
+    for (int i = 0; i < ACC; ++i) {
         const int m = tid + NT * i;
         if (m < nmodes) {
             acc_x[m] = acc0[i];

@@ -20,9 +20,18 @@ namespace shhd {
 
 constexpr int kMaxPasses = 16;
 
+// One Stockham pass: radix R applied to sub-transforms of length ns (a power of
+// two: radix-3 passes run last), m = L/R butterflies per sequence, twiddle
+// stride tstep = L/(ns R).  inv_m is 1/m for the sequence split of batched
+// transforms.
+struct FftPass {
+    int radix, m, lns, tstep;
+    float inv_m;
+};
+
 struct FftPlanDev {
     int npass;
-    int radix[kMaxPasses];
+    FftPass pass[kMaxPasses];
 };
 
 template <typename T, int R>
@@ -72,13 +81,38 @@ __device__ __forceinline__ void dft_small(cx<T> (&v)[R], int sign) {
     }
 }
 
+// w^(q t) for q = 1..R-1 from the table tw[j] = exp(-2 pi i j / L): radix 8
+// loads w^t, w^2t, w^4t and forms the other four by products (error growth of
+// at most two roundings), radix 4 loads w^t, w^2t, radix 3 and 2 load all.
+template <typename T, int R>
+__device__ __forceinline__ void apply_twiddles(cx<T> (&v)[R], const cx<T>* __restrict__ tw, int t, int sign) {
+    auto ld = [&](int e) {
+        cx<T> w = tw[e];
+        if (sign > 0) w.im = -w.im;
+        return w;
+    };
+    if constexpr (R == 8) {
+        const cx<T> w1 = ld(t), w2 = ld(2 * t), w4 = ld(4 * t);
+        const cx<T> w3 = w1 * w2, w5 = w1 * w4, w6 = w2 * w4, w7 = w3 * w4;
+        v[1] = v[1] * w1; v[2] = v[2] * w2; v[3] = v[3] * w3; v[4] = v[4] * w4;
+        v[5] = v[5] * w5; v[6] = v[6] * w6; v[7] = v[7] * w7;
+    } else if constexpr (R == 4) {
+        const cx<T> w1 = ld(t), w2 = ld(2 * t);
+        v[1] = v[1] * w1; v[2] = v[2] * w2; v[3] = v[3] * (w1 * w2);
+    } else if constexpr (R == 3) {
+        v[1] = v[1] * ld(t); v[2] = v[2] * ld(2 * t);
+    } else {
+        v[1] = v[1] * ld(t);
+    }
+}
+
 template <typename T, int NT, int VPT, int R>
-__device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, int ns, const cx<T>* __restrict__ tw,
+__device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftPass& ps, const cx<T>* __restrict__ tw,
                                          int sign) {
     constexpr int B = (VPT + R - 1) / R;
-    const int m = L / R;
+    const int m = ps.m;
     const int nb = nseq * m;
-    const int tstep = L / (ns * R);
+    const int ns_mask = (1 << ps.lns) - 1;
     cx<T> v[B][R];
     int jj[B], base[B];
 #pragma unroll
@@ -87,7 +121,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, int ns, co
         jj[b] = 0;
         base[b] = 0;
         if (id < nb) {
-            const int s = id / m;
+            const int s = (nseq == 1) ? 0 : static_cast<int>((static_cast<float>(id) + 0.5f) * ps.inv_m);
             const int j = id - s * m;
             jj[b] = j;
             base[b] = s * L;
@@ -101,19 +135,12 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, int ns, co
         const int id = static_cast<int>(threadIdx.x) + NT * b;
         if (id < nb) {
             const int j = jj[b];
-            const int k = j % ns;
-            if (k != 0) {
-#pragma unroll
-                for (int q = 1; q < R; ++q) {
-                    cx<T> w = tw[q * k * tstep];
-                    if (sign > 0) w.im = -w.im;
-                    v[b][q] = v[b][q] * w;
-                }
-            }
+            const int k = j & ns_mask;
+            if (k != 0) apply_twiddles<T, R>(v[b], tw, k * ps.tstep, sign);
             dft_small<T, R>(v[b], sign);
-            const int out = base[b] + (j / ns) * ns * R + k;
+            const int out = base[b] + ((j >> ps.lns) << ps.lns) * R + k;
 #pragma unroll
-            for (int p = 0; p < R; ++p) buf[out + p * ns] = v[b][p];
+            for (int p = 0; p < R; ++p) buf[out + (p << ps.lns)] = v[b][p];
         }
     }
     __syncthreads();
@@ -124,14 +151,12 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, int ns, co
 template <typename T, int NT, int VPT>
 __device__ __forceinline__ void lds_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
                                         const cx<T>* __restrict__ tw, int sign) {
-    int ns = 1;
     for (int p = 0; p < plan.npass; ++p) {
-        const int r = plan.radix[p];
-        if (r == 8) fft_pass<T, NT, VPT, 8>(buf, L, nseq, ns, tw, sign);
-        else if (r == 4) fft_pass<T, NT, VPT, 4>(buf, L, nseq, ns, tw, sign);
-        else if (r == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ns, tw, sign);
-        else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ns, tw, sign);
-        ns *= r;
+        const FftPass& ps = plan.pass[p];
+        if (ps.radix == 8) fft_pass<T, NT, VPT, 8>(buf, L, nseq, ps, tw, sign);
+        else if (ps.radix == 4) fft_pass<T, NT, VPT, 4>(buf, L, nseq, ps, tw, sign);
+        else if (ps.radix == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ps, tw, sign);
+        else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ps, tw, sign);
     }
 }
 

@@ -21,7 +21,7 @@
 #include <vector>
 
 #include "../../include/shhd_gpu.h"
-#include "kernels.h"
+#include "instances.h"
 
 using namespace shhd;
 
@@ -35,6 +35,7 @@ constexpr size_t kLdsCap3 = 128 * 1024;
 
 struct ColPlan {
     int C = 1, NT = 256, V = 6, ntiles = 1;
+    int acc = 6;       // K3 accumulator slots per lane: 4 when C*Nay <= 4*NT, else V
     size_t lds = 0;    // K1: C*My values
     size_t lds3 = 0;   // K3: max(C*My, 2*C*Nay) values (FFT column, then both accumulators)
 };
@@ -43,20 +44,35 @@ struct RowPlan {
     size_t lds = 0;
 };
 
-bool is_2a3b(int n) {
+// device transform lengths: 2^a or 3 * 2^a (every Stockham sub-length ns is
+// then a power of two, which the pass indexing relies on)
+bool is_device_length(int n) {
     if (n <= 0) return false;
+    if (n % 3 == 0) n /= 3;
     while (n % 2 == 0) n /= 2;
-    while (n % 3 == 0) n /= 3;
     return n == 1;
 }
 
 FftPlanDev make_fft_plan(int L) {
     FftPlanDev p{};
+    std::vector<int> rad;
     int m = L;
-    while (m % 8 == 0 && m > 0) { p.radix[p.npass++] = 8; m /= 8; }
-    while (m % 4 == 0) { p.radix[p.npass++] = 4; m /= 4; }
-    while (m % 2 == 0) { p.radix[p.npass++] = 2; m /= 2; }
-    while (m % 3 == 0) { p.radix[p.npass++] = 3; m /= 3; }
+    while (m % 8 == 0 && m > 0) { rad.push_back(8); m /= 8; }
+    while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+    while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+    while (m % 3 == 0) { rad.push_back(3); m /= 3; }   // radix 3 last: every ns stays a power of two
+    int ns = 1;
+    for (int r : rad) {
+        FftPass& ps = p.pass[p.npass++];
+        ps.radix = r;
+        ps.m = L / r;
+        int lns = 0;
+        while ((1 << lns) < ns) ++lns;
+        ps.lns = lns;
+        ps.tstep = L / (ns * r);
+        ps.inv_m = 1.0f / static_cast<float>(ps.m);
+        ns *= r;
+    }
     return p;
 }
 
@@ -76,8 +92,9 @@ ColPlan plan_columns(int my, int nay, int hxa, size_t elem) {
     while (C > 1 && (lds3(C) > kLdsCap3 || C * my > 1024 * p.V)) C >>= 1;
     p.C = C;
     const int n = C * my;
-    p.NT = (n <= 64 * p.V) ? 64 : (n <= 256 * p.V ? 256 : 1024);
+    p.NT = (n <= 64 * p.V) ? 64 : (n <= 256 * p.V ? 256 : (n <= 512 * p.V ? 512 : 1024));
     p.ntiles = (hxa + C - 1) / C;
+    p.acc = (C * nay <= 4 * p.NT) ? 4 : p.V;
     p.lds = static_cast<size_t>(C) * my * elem;
     p.lds3 = lds3(C);
     return p;
@@ -86,7 +103,7 @@ ColPlan plan_columns(int my, int nay, int hxa, size_t elem) {
 RowPlan plan_rows(int mx, int my, size_t elem) {
     RowPlan p;
     p.V = (mx % 3 == 0) ? 6 : 8;
-    p.NT = (mx <= 64 * p.V) ? 64 : (mx <= 256 * p.V ? 256 : 1024);
+    p.NT = (mx <= 64 * p.V) ? 64 : (mx <= 256 * p.V ? 256 : (mx <= 512 * p.V ? 512 : 1024));
     p.R = std::max(1, (p.NT * p.V) / mx);
     while (p.R > 1 && static_cast<size_t>(p.R) * mx * elem > kLdsCap) --p.R;
     p.R = std::min(p.R, my);
@@ -108,38 +125,29 @@ using K3Fn = void (*)(Desc, StagePtrs<T>, StageParams, const cx<T>*, const cx<T>
 template <typename T>
 K1Fn<T> pick_k1(int NT, int V) {
 #define SHHD_K1(nt, v) if (NT == nt && V == v) return &k_inverse_columns<T, nt, v>;
-    SHHD_K1(64, 6) SHHD_K1(64, 8) SHHD_K1(256, 6) SHHD_K1(256, 8) SHHD_K1(1024, 6) SHHD_K1(1024, 8)
+    SHHD_FOR_NT_V(SHHD_K1)
 #undef SHHD_K1
     return nullptr;
 }
 
 template <typename T>
-K3Fn<T> pick_k3(int NT, int V) {
-#define SHHD_K3(nt, v) if (NT == nt && V == v) return &k_forward_columns<T, nt, v>;
-    SHHD_K3(64, 6) SHHD_K3(64, 8) SHHD_K3(256, 6) SHHD_K3(256, 8) SHHD_K3(1024, 6) SHHD_K3(1024, 8)
+K3Fn<T> pick_k3(int NT, int V, int acc) {
+#define SHHD_K3(nt, v) \
+    if (NT == nt && V == v && acc == 4) return &k_forward_columns<T, nt, v, 4>; \
+    if (NT == nt && V == v && acc == v) return &k_forward_columns<T, nt, v, v>;
+    SHHD_FOR_NT_V(SHHD_K3)
 #undef SHHD_K3
     return nullptr;
 }
 
-template <typename T, int NT, int V>
-K2Fn<T> pick_k2_n(int maxn) {
-    switch (maxn) {
-        case 0: return &k_rows<T, NT, V, 0>;
-        case 1: return &k_rows<T, NT, V, 1>;
-        case 2: return &k_rows<T, NT, V, 2>;
-        case 4: return &k_rows<T, NT, V, 4>;
-        default: return &k_rows<T, NT, V, 8>;
-    }
-}
-
 template <typename T>
 K2Fn<T> pick_k2(int NT, int V, int maxn) {
-    if (NT == 64 && V == 6) return pick_k2_n<T, 64, 6>(maxn);
-    if (NT == 64 && V == 8) return pick_k2_n<T, 64, 8>(maxn);
-    if (NT == 256 && V == 6) return pick_k2_n<T, 256, 6>(maxn);
-    if (NT == 256 && V == 8) return pick_k2_n<T, 256, 8>(maxn);
-    if (NT == 1024 && V == 6) return pick_k2_n<T, 1024, 6>(maxn);
-    if (NT == 1024 && V == 8) return pick_k2_n<T, 1024, 8>(maxn);
+    const int n = maxn <= 0 ? 0 : (maxn == 1 ? 1 : (maxn == 2 ? 2 : (maxn <= 4 ? 4 : 8)));
+#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn>;
+#define SHHD_K2(nt, v) SHHD_K2N(nt, v, 0) SHHD_K2N(nt, v, 1) SHHD_K2N(nt, v, 2) SHHD_K2N(nt, v, 4) SHHD_K2N(nt, v, 8)
+    SHHD_FOR_NT_V(SHHD_K2)
+#undef SHHD_K2
+#undef SHHD_K2N
     return nullptr;
 }
 
@@ -516,8 +524,8 @@ shhd_status validate(const shhd_config& c) {
             return set_err(nullptr, SHHD_ERR_INVALID, "fix force/gradient direction must be x|y|0|1.");
     }
     // device transform shapes
-    if (!is_2a3b(c.nx) || !is_2a3b(c.ny))
-        return set_err(nullptr, SHHD_ERR_UNSUPPORTED, "compute grid sides must be of the form 2^a 3^b on the device");
+    if (!is_device_length(c.nx) || !is_device_length(c.ny))
+        return set_err(nullptr, SHHD_ERR_UNSUPPORTED, "compute grid sides must be 2^a or 3*2^a on the device");
     const int vx = (c.nx % 3 == 0) ? 6 : 8, vy = (c.ny % 3 == 0) ? 6 : 8;
     if (c.nx > 1024 * vx || c.ny > 1024 * vy)
         return set_err(nullptr, SHHD_ERR_UNSUPPORTED, "compute grid side exceeds the device transform limit (6144 or 8192)");
@@ -598,7 +606,7 @@ shhd_status run_stage(shhd_gpu* h, int next, int cur, int base, const StageParam
         });
         if (st != SHHD_OK) return st;
     }
-    K3Fn<T> k3 = pick_k3<T>(cp.NT, cp.V);
+    K3Fn<T> k3 = pick_k3<T>(cp.NT, cp.V, cp.acc);
     return timed_launch(h, 2, [&] {
         hipLaunchKernelGGL(k3, dim3(cp.ntiles, d.ngroups), dim3(cp.NT), cp.lds3, h->stream, d, P, sp,
                            static_cast<const cx<T>*>(ub), twy, cp.C);
@@ -611,7 +619,7 @@ shhd_status set_lds_limits(shhd_gpu* h) {
     const RowPlan& rp = h->rows;
     SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(cp.NT, cp.V)),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(cp.lds)));
-    SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(cp.NT, cp.V)),
+    SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(cp.NT, cp.V, cp.acc)),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(cp.lds3)));
     SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(rp.NT, rp.V, rp.maxn)),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rp.lds)));

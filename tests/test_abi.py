@@ -43,7 +43,8 @@ def test_config_struct_layout_matches_header():
     (dict(nx_active=7, dealias="none"), "grid must be even"),
     (dict(mode=shhd.MODE_QUIESCENT, momentum_advection=True), "quiescent time evolution cannot use momentum"),
     (dict(dt=-1.0), "timestep must be positive"),
-    (dict(nx_active=10, ny_active=10, dealias="none"), "2^a 3^b"),
+    (dict(nx_active=10, ny_active=10, dealias="none"), "2^a or 3*2^a"),
+    (dict(nx_active=96, ny_active=96, dealias="3/2"), "2^a or 3*2^a"),
     (dict(eta=-1.0), "nonnegative eta"),
 ])
 def test_invalid_config_rejected_before_device(kw, msg):
