@@ -55,7 +55,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(Desc d, StagePtrs<T> P, 
             const int cidx = ky >= 0 ? ky : ky + d.nay;
             v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
         }
-        lds[idx] = v;
+        lds[lp(idx)] = v;
     }
     __syncthreads();
     lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, +1);
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(Desc d, StagePtrs<T> P, 
         const int y = idx / C;
         const int c = idx - y * C;
         const int gx = gx0 + c;
-        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = lds[c * L + y];
+        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = lds[lp(c * L + y)];
     }
 }
 
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ t
                     z = mk(a.re + b.im, b.re - a.im);               // conj(A) + i conj(B)
                 }
             }
-            lds[idx] = z;
+            lds[lp(idx)] = z;
         }
         __syncthreads();
         lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, +1);
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ t
         for (int i = 0; i < PPT; ++i) {
             const int idx = tid + NT * i;
             if (idx < npts) {
-                const cx<T> z = lds[idx];
+                const cx<T> z = lds[lp(idx)];
                 F.store(ra, i, z.re);
                 if (rb >= 0) F.store(rb, i, z.im);
             }
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ t
             if (idx < npts) {
                 const T va = F.eval(d, ia, i);
                 const T vb = has_b ? F.eval(d, ib, i) : T(0);
-                lds[idx] = mk(va, vb);
+                lds[lp(idx)] = mk(va, vb);
             }
         }
         __syncthreads();
@@ -220,8 +220,8 @@ __global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ t
             const int k = idx - r * hxa;
             const int y = y0 + r;
             if (y < my) {
-                const cx<T> zk = lds[r * L + k];
-                const cx<T> zm = lds[r * L + (k == 0 ? 0 : L - k)];
+                const cx<T> zk = lds[lp(r * L + k)];
+                const cx<T> zm = lds[lp(r * L + (k == 0 ? 0 : L - k))];
                 const size_t at = static_cast<size_t>(y) * hxa + k;
                 ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
                 if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, 
             const int y = idx / C;
             const int c = idx - y * C;
             const int gx = gx0 + c;
-            lds[c * L + y] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
+            lds[lp(c * L + y)] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
         }
         __syncthreads();
         lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, 
             if (gx >= hxa || cidx == half) continue;
             const int ky_i = cidx < half ? cidx : cidx - nay;
             const int gy = ky_i >= 0 ? ky_i : ky_i + L;
-            const cx<T> v = lds[c * L + gy];
+            const cx<T> v = lds[lp(c * L + gy)];
             const T kx = T(two_pi * gx / d.lx);
             const T ky = T(two_pi * ky_i / d.ly);
             switch (kind) {

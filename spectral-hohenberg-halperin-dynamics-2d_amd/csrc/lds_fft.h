@@ -20,6 +20,12 @@ namespace shhd {
 
 constexpr int kMaxPasses = 16;
 
+// LDS slot of FFT element i: one pad slot after every 16 elements, so the
+// stride-R writes of the early Stockham passes spread over the banks.
+__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+// LDS values needed for n padded elements
+__host__ __device__ __forceinline__ int lp_size(int n) { return n + (n >> 4) + 1; }
+
 // One Stockham pass: radix R applied to sub-transforms of length ns (a power of
 // two: radix-3 passes run last), m = L/R butterflies per sequence, twiddle
 // stride tstep = L/(ns R).  inv_m is 1/m for the sequence split of batched
@@ -126,7 +132,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
             jj[b] = j;
             base[b] = s * L;
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[b][q] = buf[s * L + j + q * m];
+            for (int q = 0; q < R; ++q) v[b][q] = buf[lp(s * L + j + q * m)];
         }
     }
     __syncthreads();
@@ -140,7 +146,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
             dft_small<T, R>(v[b], sign);
             const int out = base[b] + ((j >> ps.lns) << ps.lns) * R + k;
 #pragma unroll
-            for (int p = 0; p < R; ++p) buf[out + (p << ps.lns)] = v[b][p];
+            for (int p = 0; p < R; ++p) buf[lp(out + (p << ps.lns))] = v[b][p];
         }
     }
     __syncthreads();

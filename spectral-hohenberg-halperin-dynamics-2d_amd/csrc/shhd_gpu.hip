@@ -85,17 +85,17 @@ int pow2ceil(int v) {
 ColPlan plan_columns(int my, int nay, int hxa, size_t elem) {
     ColPlan p;
     p.V = (my % 3 == 0) ? 6 : 8;
-    int target = static_cast<int>(64 / elem);   // 64-byte row segments
+    int target = static_cast<int>(32 / elem);   // 32-byte row segments (measured best at 1536^2)
     if (const char* e = std::getenv("SHHD_COLUMN_TILE")) target = std::max(1, std::atoi(e));   // tuning override
     int C = std::min(std::max(target, 1), pow2ceil(hxa));
-    auto lds3 = [&](int c) { return static_cast<size_t>(c) * std::max(my, 2 * nay) * elem; };
+    auto lds3 = [&](int c) { return static_cast<size_t>(std::max(lp_size(c * my), 2 * c * nay)) * elem; };
     while (C > 1 && (lds3(C) > kLdsCap3 || C * my > 1024 * p.V)) C >>= 1;
     p.C = C;
     const int n = C * my;
     p.NT = (n <= 64 * p.V) ? 64 : (n <= 256 * p.V ? 256 : (n <= 512 * p.V ? 512 : 1024));
     p.ntiles = (hxa + C - 1) / C;
     p.acc = (C * nay <= 4 * p.NT) ? 4 : p.V;
-    p.lds = static_cast<size_t>(C) * my * elem;
+    p.lds = static_cast<size_t>(lp_size(C * my)) * elem;
     p.lds3 = lds3(C);
     return p;
 }
@@ -105,10 +105,10 @@ RowPlan plan_rows(int mx, int my, size_t elem) {
     p.V = (mx % 3 == 0) ? 6 : 8;
     p.NT = (mx <= 64 * p.V) ? 64 : (mx <= 256 * p.V ? 256 : (mx <= 512 * p.V ? 512 : 1024));
     p.R = std::max(1, (p.NT * p.V) / mx);
-    while (p.R > 1 && static_cast<size_t>(p.R) * mx * elem > kLdsCap) --p.R;
+    while (p.R > 1 && static_cast<size_t>(lp_size(p.R * mx)) * elem > kLdsCap) --p.R;
     p.R = std::min(p.R, my);
     p.nblocks = (my + p.R - 1) / p.R;
-    p.lds = static_cast<size_t>(p.R) * mx * elem;
+    p.lds = static_cast<size_t>(lp_size(p.R * mx)) * elem;
     return p;
 }
 
