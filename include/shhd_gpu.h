@@ -44,7 +44,8 @@ typedef enum shhd_status {
     SHHD_ERR_UNSUPPORTED = 2,  /* valid for the reference, no device functor   */
     SHHD_ERR_HIP = 3,          /* HIP runtime error / device unavailable       */
     SHHD_ERR_ZERO_DENSITY = 4, /* "compressible velocity encountered zero density." */
-    SHHD_ERR_RUNTIME = 5       /* other runtime error raised by the physics    */
+    SHHD_ERR_RUNTIME = 5,      /* other runtime error raised by the physics    */
+    SHHD_ERR_FULL = 6          /* batch: device budget or instance slots used up */
 } shhd_status;
 
 /* dynamics mode: src/fcalculator_dynamics_mode.h:13-30 */
@@ -173,6 +174,34 @@ shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long*
  * model: compact spectra s*Na^2 per field, column/row intermediates
  * 1.5*s*Na^2 per field under 3/2 padding). */
 shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int nclasses);
+
+/* ---- batched instances ------------------------------------------------------
+ * No single reference counterpart: the reference runs ensembles as independent
+ * one-rank processes (examples/05_yokota_green_kubo/job_ohtaka_linear_samples.sh:26-36,
+ * dev_env/ohtaka/README.md:283-293).  A batch keeps many simulations resident
+ * in HBM and advances all of them with one launch per kernel and stage for each
+ * launch shape; every instance computes exactly what it would compute alone
+ * (same kernels, per-instance Philox counters), whatever else is in the batch.
+ * Instances may differ in grid, model, integrator and precision.
+ */
+typedef struct shhd_batch shhd_batch;
+
+/* device_budget_bytes = 0: no budget.  use_graphs: replay captured 3-step
+ * hipGraphs for runs of >= 3 steps. */
+shhd_status shhd_batch_create(size_t device_budget_bytes, int use_graphs, shhd_batch** out);
+/* device bytes an instance of this configuration would hold */
+shhd_status shhd_batch_footprint(const shhd_config* config, size_t* bytes);
+/* admit one simulation with its initial reference-layout state; SHHD_ERR_FULL
+ * when the budget or the slots are used up */
+shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const double* state, size_t n, int* slot);
+/* advance every admitted instance by nsteps */
+shhd_status shhd_batch_step(shhd_batch* b, int nsteps);
+shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n);
+/* free an instance's device memory and slot */
+shhd_status shhd_batch_retire(shhd_batch* b, int slot);
+shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes);
+void        shhd_batch_destroy(shhd_batch* b);
+const char* shhd_batch_last_error(const shhd_batch* b);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,5 @@
-// desc.h — the per-handle descriptor that the host lowers from shhd_config and
-// passes to every kernel by value (kernel-argument segment, read through the
-// scalar unit).
+// desc.h — the per-instance descriptor that the host lowers from shhd_config,
+// and the device-side tables through which one launch serves many instances.
 #pragma once
 
 #include "lds_fft.h"
@@ -69,7 +68,6 @@ struct Desc {
     // row pass
     int make_velocity;          // v = j / rho (compressible) or j / rho0
     int velocity_compressible;
-    double inv_rho0;
     int n_out;
     OutItem out[kMaxOut];
     double phi4_u[kMaxOp];
@@ -121,7 +119,30 @@ struct StagePtrs {
 struct StageParams {
     double beta, wb, wc;
     int use_b;              // SRK3: A + beta B; Euler: A only
-    uint32_t step_lo, step_hi;
+    uint32_t step_lo, step_hi;   // filled in-kernel from the instance's step counter
+};
+
+// Everything one instance needs on the device, kept in a device-side table so
+// that one launch can serve many instances of different sizes (batched
+// ensembles): the workgroups of a launch are mapped to (instance, local block)
+// through BlockMap.
+template <typename T>
+struct InstDev {
+    Desc d;
+    const cx<T>* twx;         // twiddles exp(-2 pi i j / Mx), j < Mx
+    const cx<T>* twy;         // same for My
+    cx<T>* tbuf;              // K1 -> K2 planes
+    cx<T>* ubuf;              // K2 -> K3 planes
+    int* flag;                // zero-density flag
+    const uint64_t* step;     // Philox step counter of this instance
+    int C;                    // kx columns per column-pass workgroup
+    int R;                    // rows per row-pass workgroup
+    int ntiles;               // column tiles = ceil(Nax/2 / C)
+};
+
+struct BlockMap {
+    const int* inst;          // instance of each workgroup of the launch
+    const int* first;         // first workgroup of each instance
 };
 
 }  // namespace shhd

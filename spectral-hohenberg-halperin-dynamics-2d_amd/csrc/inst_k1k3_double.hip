@@ -3,8 +3,8 @@
 #include "instances.h"
 namespace shhd {
 #define SHHD_INST(nt, v) \
-    template __global__ void k_inverse_columns<double, nt, v>(Desc, StagePtrs<double>, cx<double>*, const cx<double>*, int); \
-    template __global__ void k_forward_columns<double, nt, v, 4>(Desc, StagePtrs<double>, StageParams, const cx<double>*, const cx<double>*, int); \
-    template __global__ void k_forward_columns<double, nt, v, v>(Desc, StagePtrs<double>, StageParams, const cx<double>*, const cx<double>*, int);
+    template __global__ void k_inverse_columns<double, nt, v>(const InstDev<double>*, const StagePtrs<double>*, BlockMap); \
+    template __global__ void k_forward_columns<double, nt, v, 4>(const InstDev<double>*, const StagePtrs<double>*, StageParams, BlockMap); \
+    template __global__ void k_forward_columns<double, nt, v, v>(const InstDev<double>*, const StagePtrs<double>*, StageParams, BlockMap);
 SHHD_FOR_NT_V(SHHD_INST)
 }  // namespace shhd

@@ -3,7 +3,7 @@
 #include "instances.h"
 namespace shhd {
 #define SHHD_INST_N(nt, v, mn) \
-    template __global__ void k_rows<double, nt, v, mn>(Desc, const cx<double>*, cx<double>*, const cx<double>*, int, int*);
+    template __global__ void k_rows<double, nt, v, mn>(const InstDev<double>*, BlockMap);
 #define SHHD_INST(nt, v) SHHD_INST_N(nt, v, 0) SHHD_INST_N(nt, v, 1) SHHD_INST_N(nt, v, 2) SHHD_INST_N(nt, v, 4) SHHD_INST_N(nt, v, 8)
 SHHD_FOR_NT_V(SHHD_INST)
 }  // namespace shhd

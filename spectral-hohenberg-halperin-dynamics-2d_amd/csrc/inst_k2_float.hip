@@ -3,7 +3,7 @@
 #include "instances.h"
 namespace shhd {
 #define SHHD_INST_N(nt, v, mn) \
-    template __global__ void k_rows<float, nt, v, mn>(Desc, const cx<float>*, cx<float>*, const cx<float>*, int, int*);
+    template __global__ void k_rows<float, nt, v, mn>(const InstDev<float>*, BlockMap);
 #define SHHD_INST(nt, v) SHHD_INST_N(nt, v, 0) SHHD_INST_N(nt, v, 1) SHHD_INST_N(nt, v, 2) SHHD_INST_N(nt, v, 4) SHHD_INST_N(nt, v, 8)
 SHHD_FOR_NT_V(SHHD_INST)
 }  // namespace shhd

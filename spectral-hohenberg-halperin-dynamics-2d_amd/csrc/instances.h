@@ -11,12 +11,11 @@
 namespace shhd {
 
 #define SHHD_DECL_K1(T, nt, v) \
-    extern template __global__ void k_inverse_columns<T, nt, v>(Desc, StagePtrs<T>, cx<T>*, const cx<T>*, int);
-#define SHHD_DECL_K3(T, nt, v, a)                                                                              \
-    extern template __global__ void k_forward_columns<T, nt, v, a>(Desc, StagePtrs<T>, StageParams, const cx<T>*, \
-                                                                   const cx<T>*, int);
+    extern template __global__ void k_inverse_columns<T, nt, v>(const InstDev<T>*, const StagePtrs<T>*, BlockMap);
+#define SHHD_DECL_K3(T, nt, v, a) \
+    extern template __global__ void k_forward_columns<T, nt, v, a>(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
 #define SHHD_DECL_K2(T, nt, v, mn) \
-    extern template __global__ void k_rows<T, nt, v, mn>(Desc, const cx<T>*, cx<T>*, const cx<T>*, int, int*);
+    extern template __global__ void k_rows<T, nt, v, mn>(const InstDev<T>*, BlockMap);
 
 #define SHHD_DECL_ALL(T, nt, v)                                                              \
     SHHD_DECL_K1(T, nt, v) SHHD_DECL_K3(T, nt, v, 4) SHHD_DECL_K3(T, nt, v, v)                \

@@ -34,13 +34,21 @@ __device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2)
 // K1
 // ---------------------------------------------------------------------------
 template <typename T, int NT, int VPT>
-__global__ __launch_bounds__(NT) void k_inverse_columns(Desc d, StagePtrs<T> P, cx<T>* __restrict__ tbuf,
-                                                        const cx<T>* __restrict__ twy, int C) {
+__global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __restrict__ insts,
+                                                        const StagePtrs<T>* __restrict__ ptrs, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const int slot = blockIdx.y;
+    const int inst = map.inst[blockIdx.x];
+    const int local = blockIdx.x - map.first[inst];
+    const InstDev<T>& I = insts[inst];
+    const Desc& d = I.d;
+    const StagePtrs<T>& P = ptrs[inst];
+    const int C = I.C;
+    const cx<T>* __restrict__ twy = I.twy;
+    cx<T>* __restrict__ tbuf = I.tbuf;
+    const int slot = local / I.ntiles;
     const cx<T>* __restrict__ src = P.cur[d.in_field[slot]];
-    const int gx0 = blockIdx.x * C;
+    const int gx0 = (local - slot * I.ntiles) * C;
     const int L = d.my;
     const int ncol = C * L;
     const int half = d.nay / 2;
@@ -120,14 +128,22 @@ struct RowFields {
 };
 
 template <typename T, int NT, int PPT, int MAXN>
-__global__ __launch_bounds__(NT) void k_rows(Desc d, const cx<T>* __restrict__ tbuf, cx<T>* __restrict__ ubuf,
-                                             const cx<T>* __restrict__ twx, int R, int* __restrict__ flag) {
+__global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int inst = map.inst[blockIdx.x];
+    const int local = blockIdx.x - map.first[inst];
+    const InstDev<T>& I = insts[inst];
+    const Desc& d = I.d;
+    const cx<T>* __restrict__ tbuf = I.tbuf;
+    cx<T>* __restrict__ ubuf = I.ubuf;
+    const cx<T>* __restrict__ twx = I.twx;
+    int* __restrict__ flag = I.flag;
+    const int R = I.R;
     const int L = d.mx;
     const int my = d.my;
     const int hxa = d.hxa;
-    const int y0 = blockIdx.x * R;
+    const int y0 = local * R;
     const int npts = R * L;
     const size_t plane = static_cast<size_t>(my) * hxa;
     const int tid = threadIdx.x;
@@ -279,17 +295,31 @@ __device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<dou
 // K3
 // ---------------------------------------------------------------------------
 template <typename T, int NT, int VPT, int ACC>
-__global__ __launch_bounds__(NT) void k_forward_columns(Desc d, StagePtrs<T> P, StageParams sp,
-                                                        const cx<T>* __restrict__ ubuf,
-                                                        const cx<T>* __restrict__ twy, int C) {
+__global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __restrict__ insts,
+                                                        const StagePtrs<T>* __restrict__ ptrs, StageParams sp,
+                                                        BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const Group& g = d.groups[blockIdx.y];
+    const int inst = map.inst[blockIdx.x];
+    const int local = blockIdx.x - map.first[inst];
+    const InstDev<T>& I = insts[inst];
+    const Desc& d = I.d;
+    const StagePtrs<T>& P = ptrs[inst];
+    const int C = I.C;
+    const cx<T>* __restrict__ ubuf = I.ubuf;
+    const cx<T>* __restrict__ twy = I.twy;
+    {
+        const uint64_t st = *I.step;
+        sp.step_lo = static_cast<uint32_t>(st);
+        sp.step_hi = static_cast<uint32_t>(st >> 32);
+    }
+    const int group = local / I.ntiles;
+    const Group& g = d.groups[group];
     const int L = d.my;
     const int nay = d.nay;
     const int half = nay / 2;
     const int hxa = d.hxa;
-    const int gx0 = blockIdx.x * C;
+    const int gx0 = (local - group * I.ntiles) * C;
     const int ncol = C * L;
     const int nmodes = C * nay;
     const size_t plane = static_cast<size_t>(L) * hxa;

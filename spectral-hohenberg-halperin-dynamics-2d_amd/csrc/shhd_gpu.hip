@@ -1,14 +1,19 @@
-// shhd_gpu.hip — C-ABI of the MI355X right-hand-side and time-step engine
-// (declared in include/shhd_gpu.h).
+// shhd_gpu.hip — host side of the MI355X right-hand-side and time-step engine:
+// descriptor lowering, launch planning, device tables, stepping (eager or
+// replayed from a captured hipGraph) and the C-ABI of include/shhd_gpu.h.
 //
-// One handle = one simulation on one device stream.  The state lives in HBM in
-// a compact spectral layout holding active modes only:
+// Every simulation is an Instance.  Its state lives in HBM in a compact
+// spectral layout holding active modes only:
 //     field f, column gx < Nax/2, slot c < Nay  ->  index f*(Nax/2*Nay) + gx*Nay + c
 // with ky = c for c < Nay/2 and ky = c - Nay otherwise (slot Nay/2 is the
-// always-zero ky = -Nay/2 mode).  Per stage the engine runs
+// always-zero ky = -Nay/2 mode).  Per stage three kernels run:
 //     K1 (inverse y, F_in fields) -> K2 (rows: x transforms + products)
-//     -> K3 (forward y + spectral RHS + noise + stage update),
-// so one SRK3 step is 9 launches and one Euler step 3.
+//     -> K3 (forward y + spectral RHS + noise + stage update).
+// An Engine holds any number of instances; instances with the same launch
+// shape form a class and each kernel of a stage is ONE launch per class, the
+// workgroups being mapped to (instance, local block) through device tables.
+// A single-simulation handle (shhd_gpu) is an engine with one instance; the
+// batch API (shhd_batch) exposes the engine for ensembles and serving.
 
 #include <hip/hip_runtime.h>
 
@@ -17,7 +22,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/shhd_gpu.h"
@@ -32,12 +40,13 @@ std::string g_create_error;
 constexpr int kClasses = 3;
 constexpr size_t kLdsCap = 96 * 1024;
 constexpr size_t kLdsCap3 = 128 * 1024;
+constexpr int kMaxSlots = 4096;
 
 struct ColPlan {
     int C = 1, NT = 256, V = 6, ntiles = 1;
     int acc = 6;       // K3 accumulator slots per lane: 4 when C*Nay <= 4*NT, else V
-    size_t lds = 0;    // K1: C*My values
-    size_t lds3 = 0;   // K3: max(C*My, 2*C*Nay) values (FFT column, then both accumulators)
+    size_t lds = 0;    // K1: padded C*My values
+    size_t lds3 = 0;   // K3: max(padded C*My, 2*C*Nay) values (FFT column, then both accumulators)
 };
 struct RowPlan {
     int R = 1, NT = 256, V = 6, nblocks = 1, maxn = 0;
@@ -82,17 +91,25 @@ int pow2ceil(int v) {
     return p;
 }
 
-ColPlan plan_columns(int my, int nay, int hxa, size_t elem) {
+int lanes_for(int n, int v, int nt_min) {
+    int nt = (n <= 64 * v) ? 64 : (n <= 256 * v ? 256 : (n <= 512 * v ? 512 : 1024));
+    return std::max(nt, nt_min);
+}
+
+// Column tile of C kx columns per workgroup.  The natural plan targets
+// 32-byte row segments; a batch plan (nt_min = 256) widens C so small grids
+// share the 256-lane shape of the larger ones.
+ColPlan plan_columns(int my, int nay, int hxa, size_t elem, int nt_min) {
     ColPlan p;
     p.V = (my % 3 == 0) ? 6 : 8;
     int target = static_cast<int>(32 / elem);   // 32-byte row segments (measured best at 1536^2)
     if (const char* e = std::getenv("SHHD_COLUMN_TILE")) target = std::max(1, std::atoi(e));   // tuning override
+    if (nt_min > 0) target = std::max(target, (nt_min * p.V) / my);
     int C = std::min(std::max(target, 1), pow2ceil(hxa));
     auto lds3 = [&](int c) { return static_cast<size_t>(std::max(lp_size(c * my), 2 * c * nay)) * elem; };
     while (C > 1 && (lds3(C) > kLdsCap3 || C * my > 1024 * p.V)) C >>= 1;
     p.C = C;
-    const int n = C * my;
-    p.NT = (n <= 64 * p.V) ? 64 : (n <= 256 * p.V ? 256 : (n <= 512 * p.V ? 512 : 1024));
+    p.NT = lanes_for(C * my, p.V, nt_min);
     p.ntiles = (hxa + C - 1) / C;
     p.acc = (C * nay <= 4 * p.NT) ? 4 : p.V;
     p.lds = static_cast<size_t>(lp_size(C * my)) * elem;
@@ -100,10 +117,10 @@ ColPlan plan_columns(int my, int nay, int hxa, size_t elem) {
     return p;
 }
 
-RowPlan plan_rows(int mx, int my, size_t elem) {
+RowPlan plan_rows(int mx, int my, size_t elem, int nt_min) {
     RowPlan p;
     p.V = (mx % 3 == 0) ? 6 : 8;
-    p.NT = (mx <= 64 * p.V) ? 64 : (mx <= 256 * p.V ? 256 : (mx <= 512 * p.V ? 512 : 1024));
+    p.NT = lanes_for(mx, p.V, nt_min);
     p.R = std::max(1, (p.NT * p.V) / mx);
     while (p.R > 1 && static_cast<size_t>(lp_size(p.R * mx)) * elem > kLdsCap) --p.R;
     p.R = std::min(p.R, my);
@@ -113,14 +130,14 @@ RowPlan plan_rows(int mx, int my, size_t elem) {
 }
 
 // ---------------------------------------------------------------------------
-// kernel dispatch over the instantiated (NT, V, MAXN) shapes
+// kernel dispatch over the instantiated (NT, V, ...) shapes
 // ---------------------------------------------------------------------------
 template <typename T>
-using K1Fn = void (*)(Desc, StagePtrs<T>, cx<T>*, const cx<T>*, int);
+using K1Fn = void (*)(const InstDev<T>*, const StagePtrs<T>*, BlockMap);
 template <typename T>
-using K2Fn = void (*)(Desc, const cx<T>*, cx<T>*, const cx<T>*, int, int*);
+using K2Fn = void (*)(const InstDev<T>*, BlockMap);
 template <typename T>
-using K3Fn = void (*)(Desc, StagePtrs<T>, StageParams, const cx<T>*, const cx<T>*, int);
+using K3Fn = void (*)(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
 
 template <typename T>
 K1Fn<T> pick_k1(int NT, int V) {
@@ -151,89 +168,59 @@ K2Fn<T> pick_k2(int NT, int V, int maxn) {
     return nullptr;
 }
 
-__global__ void k_clear_flag(int* flag) { *flag = 0; }
-
-// Hermitian projection + self-conjugate fix of the kx = 0 column of fields
-// that no stage rewrites (rho when incompressible, rho and j when quiescent);
-// idempotent, so it runs at the start of every step() call.
-template <typename T>
-__global__ void k_enforce_static(cx<T>* base, int nay, size_t field_stride, const int* fields, int nfields_static,
-                                 cx<double>* k0) {
-    const int f = fields[blockIdx.x];
-    cx<T>* col = base + static_cast<size_t>(f) * field_stride;   // gx = 0 column
-    const int half = nay / 2;
-    for (int m = threadIdx.x; m < half; m += blockDim.x) {
-        if (m == 0) {
-            col[0].im = T(0);
-            if (k0) k0[f].im = 0.0;
-        } else {
-            const cx<T> a = col[m];
-            const cx<T> b = col[nay - m];
-            const cx<T> p = T(0.5) * (a + conj(b));
-            col[m] = p;
-            col[nay - m] = conj(p);
-        }
-    }
-    (void)nfields_static;
+__global__ void k_clear_flags(int* flags, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = 0;
 }
 
-}  // namespace
+__global__ void k_advance(uint64_t* steps, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) steps[i] += 1;
+}
 
-// ---------------------------------------------------------------------------
-// handle
-// ---------------------------------------------------------------------------
-struct shhd_gpu {
-    shhd_config cfg{};
-    std::string err;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool fp32 = false;
-    size_t elem = 16;   // bytes per complex value
-
-    Desc desc{};
-    ColPlan cols;
-    RowPlan rows;
-    int n_in = 0, n_out = 0, n_updated = 0, stages = 3;
-
-    size_t field_elems = 0;   // hxa * nay
-    size_t plane_elems = 0;   // my * hxa
-    void* state[3] = {nullptr, nullptr, nullptr};
-    cx<double>* k0[3] = {nullptr, nullptr, nullptr};
-    void* tbuf = nullptr;
-    void* ubuf = nullptr;
-    void* twx = nullptr;
-    void* twy = nullptr;
-    int* dflag = nullptr;
-    int* dstatic = nullptr;
-    std::vector<int> static_fields;
-    bool is_static[kMaxFields] = {};
-    int cur = 0;
-    uint64_t step_index = 0;
-    bool state_set = false;
-    bool rho0_needed = false;
-
-    bool timing = false;
-    double time_ms[kClasses] = {0, 0, 0};
-    long long launches[kClasses] = {0, 0, 0};
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
-
-    size_t device_bytes = 0;
+// kx = 0 column of a field that no stage rewrites (rho when incompressible,
+// rho and j when quiescent): Hermitian projection + real self-conjugate mode.
+// Idempotent, so it runs at the start of every step.
+template <typename T>
+struct StaticEntry {
+    cx<T>* col;
+    cx<double>* k0;   // FP64 copy of the k = 0 mode (FP32 form) or null
+    int nay;
 };
 
-namespace {
+template <typename T>
+__global__ void k_enforce_static(const StaticEntry<T>* __restrict__ entries) {
+    const StaticEntry<T> e = entries[blockIdx.x];
+    const int half = e.nay / 2;
+    for (int m = threadIdx.x; m < half; m += blockDim.x) {
+        if (m == 0) {
+            e.col[0].im = T(0);
+            if (e.k0) e.k0->im = 0.0;
+        } else {
+            const cx<T> a = e.col[m];
+            const cx<T> b = e.col[e.nay - m];
+            const cx<T> p = T(0.5) * (a + conj(b));
+            e.col[m] = p;
+            e.col[e.nay - m] = conj(p);
+        }
+    }
+}
 
-shhd_status set_err(shhd_gpu* h, shhd_status st, const std::string& msg) {
-    if (h) h->err = msg;
+shhd_status set_err(std::string* target, shhd_status st, const std::string& msg) {
+    if (target) *target = msg;
     else g_create_error = msg;
     return st;
 }
+shhd_status set_err(std::nullptr_t, shhd_status st, const std::string& msg) {
+    g_create_error = msg;
+    return st;
+}
 
-#define SHHD_HIP(h, call)                                                                      \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return set_err(h, SHHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+#define SHHD_HIP(errp, call)                                                                      \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return set_err(errp, SHHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
 template <typename T>
@@ -246,12 +233,59 @@ void make_twiddles(int L, std::vector<cx<T>>& out) {
     }
 }
 
+size_t ref_spec(const shhd_config& c) { return static_cast<size_t>(c.nx / 2 + 1) * c.ny; }
+
+bool ref_active(const shhd_config& c, int gx, int gy) {
+    if (gx < 0 || gx >= c.nx_active / 2) return false;
+    const int ky = (gy <= c.ny / 2) ? gy : gy - c.ny;
+    return std::abs(ky) < c.ny_active / 2;
+}
+
+// ---------------------------------------------------------------------------
+// one simulation
+// ---------------------------------------------------------------------------
+struct Instance {
+    shhd_config cfg{};
+    Desc desc{};
+    bool fp32 = false;
+    size_t elem = 16;
+    int stages = 3;
+    ColPlan cols;
+    RowPlan rows;
+    int n_in = 0, n_out = 0, n_updated = 0;
+    size_t field_elems = 0;   // hxa * nay
+    size_t plane_elems = 0;   // my * hxa
+    void* state[3] = {nullptr, nullptr, nullptr};
+    cx<double>* k0[3] = {nullptr, nullptr, nullptr};
+    void* tbuf = nullptr;
+    void* ubuf = nullptr;
+    void* twx = nullptr;
+    void* twy = nullptr;
+    std::vector<int> static_fields;
+    bool is_static[kMaxFields] = {};
+    bool rho0_needed = false;
+    int cur = 0;
+    uint64_t step_index = 0;
+    size_t bytes = 0;
+
+    ~Instance() {
+        for (int b = 0; b < 3; ++b) {
+            if (state[b]) (void)hipFree(state[b]);
+            if (k0[b]) (void)hipFree(k0[b]);
+        }
+        if (tbuf) (void)hipFree(tbuf);
+        if (ubuf) (void)hipFree(ubuf);
+        if (twx) (void)hipFree(twx);
+        if (twy) (void)hipFree(twy);
+    }
+};
+
 // Lower shhd_config into the kernel descriptor (see desc.h for the meaning of
 // each entry).  Mirrors the reference's request logic:
 // PhysicalRHSFieldRequests (src/fcalculator_field_requests_and_layout.h:30-127),
 // the early returns of psi_det / j_det (src/fcalculator.cc:86-150) and the
 // noise enablement (src/fluid_solver.h:79-94).
-shhd_status build_desc(shhd_gpu* h) {
+shhd_status build_desc(Instance* h) {
     const shhd_config& c = h->cfg;
     Desc& d = h->desc;
     d = Desc{};
@@ -533,211 +567,77 @@ shhd_status validate(const shhd_config& c) {
     return SHHD_OK;
 }
 
-template <typename T>
-StagePtrs<T> stage_ptrs(shhd_gpu* h, int next, int cur, int base) {
-    StagePtrs<T> p{};
-    const int nf = h->desc.nfields;
-    for (int f = 0; f < nf; ++f) {
-        auto at = [&](int b) {
-            const int bb = h->is_static[f] ? 0 : b;
-            return static_cast<cx<T>*>(h->state[bb]) + static_cast<size_t>(f) * h->field_elems;
-        };
-        p.cur[f] = at(cur);
-        p.base[f] = at(base);
-        p.next[f] = at(next);
-    }
-    if (h->fp32) {
-        p.cur_k0 = h->k0[cur];
-        p.base_k0 = h->k0[base];
-        p.next_k0 = h->k0[next];
-    }
-    return p;
+
+// bytes a configuration holds on the device (state x3, work planes, twiddles)
+size_t footprint_of(const Instance& h) {
+    const size_t nf = static_cast<size_t>(h.desc.nfields);
+    size_t b = 3 * nf * h.field_elems * h.elem;
+    if (h.fp32) b += 3 * nf * sizeof(cx<double>);
+    b += std::max<size_t>(1, static_cast<size_t>(h.n_in)) * h.plane_elems * h.elem;
+    b += std::max<size_t>(1, static_cast<size_t>(h.n_out)) * h.plane_elems * h.elem;
+    b += static_cast<size_t>(h.cfg.nx + h.cfg.ny) * h.elem;
+    return b;
 }
 
-hipEvent_t take_event(shhd_gpu* h) {
-    if (!h->ev_pool.empty()) {
-        hipEvent_t e = h->ev_pool.back();
-        h->ev_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-}
-
-template <typename F>
-shhd_status timed_launch(shhd_gpu* h, int cls, F&& launch) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (h->timing) {
-        a = take_event(h);
-        b = take_event(h);
-        SHHD_HIP(h, hipEventRecord(a, h->stream));
-    }
-    launch();
-    SHHD_HIP(h, hipGetLastError());
-    if (h->timing) {
-        SHHD_HIP(h, hipEventRecord(b, h->stream));
-        h->ev_used.push_back({cls, {a, b}});
-    }
-    return SHHD_OK;
-}
-
-template <typename T>
-shhd_status run_stage(shhd_gpu* h, int next, int cur, int base, const StageParams& sp) {
-    const Desc& d = h->desc;
-    const StagePtrs<T> P = stage_ptrs<T>(h, next, cur, base);
-    cx<T>* tb = static_cast<cx<T>*>(h->tbuf);
-    cx<T>* ub = static_cast<cx<T>*>(h->ubuf);
-    const cx<T>* twx = static_cast<const cx<T>*>(h->twx);
-    const cx<T>* twy = static_cast<const cx<T>*>(h->twy);
-    const ColPlan& cp = h->cols;
-    const RowPlan& rp = h->rows;
-    if (d.n_in > 0) {
-        K1Fn<T> k1 = pick_k1<T>(cp.NT, cp.V);
-        shhd_status st = timed_launch(h, 0, [&] {
-            hipLaunchKernelGGL(k1, dim3(cp.ntiles, d.n_in), dim3(cp.NT), cp.lds, h->stream, d, P, tb, twy, cp.C);
-        });
-        if (st != SHHD_OK) return st;
-    }
-    if (d.n_in > 0 || d.n_out > 0) {
-        K2Fn<T> k2 = pick_k2<T>(rp.NT, rp.V, rp.maxn);
-        shhd_status st = timed_launch(h, 1, [&] {
-            hipLaunchKernelGGL(k2, dim3(rp.nblocks), dim3(rp.NT), rp.lds, h->stream, d, tb, ub, twx, rp.R, h->dflag);
-        });
-        if (st != SHHD_OK) return st;
-    }
-    K3Fn<T> k3 = pick_k3<T>(cp.NT, cp.V, cp.acc);
-    return timed_launch(h, 2, [&] {
-        hipLaunchKernelGGL(k3, dim3(cp.ntiles, d.ngroups), dim3(cp.NT), cp.lds3, h->stream, d, P, sp,
-                           static_cast<const cx<T>*>(ub), twy, cp.C);
-    });
-}
-
-template <typename T>
-shhd_status set_lds_limits(shhd_gpu* h) {
-    const ColPlan& cp = h->cols;
-    const RowPlan& rp = h->rows;
-    SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(cp.NT, cp.V)),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(cp.lds)));
-    SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(cp.NT, cp.V, cp.acc)),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(cp.lds3)));
-    SHHD_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(rp.NT, rp.V, rp.maxn)),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rp.lds)));
-    return SHHD_OK;
-}
-
-template <typename T>
-shhd_status upload_twiddles(shhd_gpu* h) {
-    std::vector<cx<T>> tx, ty;
-    make_twiddles<T>(h->cfg.nx, tx);
-    make_twiddles<T>(h->cfg.ny, ty);
-    SHHD_HIP(h, hipMalloc(&h->twx, tx.size() * sizeof(cx<T>)));
-    SHHD_HIP(h, hipMalloc(&h->twy, ty.size() * sizeof(cx<T>)));
-    SHHD_HIP(h, hipMemcpy(h->twx, tx.data(), tx.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
-    SHHD_HIP(h, hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
-    h->device_bytes += (tx.size() + ty.size()) * sizeof(cx<T>);
-    return SHHD_OK;
-}
-
-size_t ref_spec(const shhd_config& c) { return static_cast<size_t>(c.nx / 2 + 1) * c.ny; }
-
-bool ref_active(const shhd_config& c, int gx, int gy) {
-    if (gx < 0 || gx >= c.nx_active / 2) return false;
-    const int ky = (gy <= c.ny / 2) ? gy : gy - c.ny;
-    return std::abs(ky) < c.ny_active / 2;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// C-ABI
-// ---------------------------------------------------------------------------
-extern "C" {
-
-shhd_status shhd_gpu_create(const shhd_config* config, shhd_gpu** out) {
-    if (!config || !out) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
-    *out = nullptr;
-    shhd_status st = validate(*config);
+// Lower and plan an instance (no device memory yet).
+shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<Instance>& out) {
+    shhd_status st = validate(cfg);
     if (st != SHHD_OK) return st;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return set_err(nullptr, SHHD_ERR_HIP, "no HIP device available");
-
-    shhd_gpu* h = new shhd_gpu;
-    h->cfg = *config;
-    h->fp32 = config->precision == SHHD_PRECISION_FP32;
+    auto h = std::make_unique<Instance>();
+    h->cfg = cfg;
+    h->fp32 = cfg.precision == SHHD_PRECISION_FP32;
     h->elem = h->fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
-    h->stages = config->integrator == SHHD_INTEGRATOR_SRK3 ? 3 : 1;
-    auto fail = [&](shhd_status s) {
-        g_create_error = h->err;
-        shhd_gpu_destroy(h);
-        return s;
-    };
-    if (hipGetDevice(&h->device) != hipSuccess) return fail(set_err(h, SHHD_ERR_HIP, "hipGetDevice failed"));
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(set_err(h, SHHD_ERR_HIP, "hipStreamCreate failed"));
-    st = build_desc(h);
-    if (st != SHHD_OK) return fail(st);
-
+    h->stages = cfg.integrator == SHHD_INTEGRATOR_SRK3 ? 3 : 1;
+    st = build_desc(h.get());
+    if (st != SHHD_OK) return st;
     const Desc& d = h->desc;
-    h->cols = plan_columns(d.my, d.nay, d.hxa, h->elem);
-    h->rows = plan_rows(d.mx, d.my, h->elem);
+    const int nt_min = batch_shape ? 256 : 0;
+    h->cols = plan_columns(d.my, d.nay, d.hxa, h->elem, nt_min);
+    h->rows = plan_rows(d.mx, d.my, h->elem, nt_min);
     int npsi_roles = 0;
     for (int i = 0; i < d.n_in; ++i)
         if (d.in_role[i] >= ROLE_PSI0) npsi_roles = std::max(npsi_roles, d.in_role[i] - ROLE_PSI0 + 1);
     h->rows.maxn = npsi_roles == 0 ? 0 : (npsi_roles == 1 ? 1 : (npsi_roles == 2 ? 2 : (npsi_roles <= 4 ? 4 : 8)));
-
     h->field_elems = static_cast<size_t>(d.hxa) * d.nay;
     h->plane_elems = static_cast<size_t>(d.my) * d.hxa;
-    const size_t nf = static_cast<size_t>(d.nfields);
-    for (int b = 0; b < 3; ++b) {
-        if (hipMalloc(&h->state[b], nf * h->field_elems * h->elem) != hipSuccess)
-            return fail(set_err(h, SHHD_ERR_HIP, "hipMalloc(state) failed"));
-        if (hipMemset(h->state[b], 0, nf * h->field_elems * h->elem) != hipSuccess)
-            return fail(set_err(h, SHHD_ERR_HIP, "hipMemset(state) failed"));
-        h->device_bytes += nf * h->field_elems * h->elem;
-        if (h->fp32) {
-            if (hipMalloc(&h->k0[b], nf * sizeof(cx<double>)) != hipSuccess)
-                return fail(set_err(h, SHHD_ERR_HIP, "hipMalloc(k0) failed"));
-            if (hipMemset(h->k0[b], 0, nf * sizeof(cx<double>)) != hipSuccess)
-                return fail(set_err(h, SHHD_ERR_HIP, "hipMemset(k0) failed"));
-        }
-    }
-    const size_t tbytes = std::max<size_t>(1, static_cast<size_t>(d.n_in)) * h->plane_elems * h->elem;
-    const size_t ubytes = std::max<size_t>(1, static_cast<size_t>(d.n_out)) * h->plane_elems * h->elem;
-    if (hipMalloc(&h->tbuf, tbytes) != hipSuccess || hipMalloc(&h->ubuf, ubytes) != hipSuccess)
-        return fail(set_err(h, SHHD_ERR_HIP, "hipMalloc(work) failed"));
-    h->device_bytes += tbytes + ubytes;
-    if (hipMalloc(&h->dflag, sizeof(int)) != hipSuccess) return fail(set_err(h, SHHD_ERR_HIP, "hipMalloc(flag) failed"));
-    if (!h->static_fields.empty()) {
-        if (hipMalloc(&h->dstatic, h->static_fields.size() * sizeof(int)) != hipSuccess)
-            return fail(set_err(h, SHHD_ERR_HIP, "hipMalloc(static) failed"));
-        if (hipMemcpy(h->dstatic, h->static_fields.data(), h->static_fields.size() * sizeof(int),
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return fail(set_err(h, SHHD_ERR_HIP, "hipMemcpy(static) failed"));
-    }
-    st = h->fp32 ? upload_twiddles<float>(h) : upload_twiddles<double>(h);
-    if (st != SHHD_OK) return fail(st);
-    st = h->fp32 ? set_lds_limits<float>(h) : set_lds_limits<double>(h);
-    if (st != SHHD_OK) return fail(st);
-    if (std::getenv("SHHD_VERBOSE")) {
-        std::fprintf(stderr,
-                     "[shhd] Mx=%d My=%d Na=%dx%d N=%d fp%d F_in=%d F_out=%d groups=%d | cols C=%d NT=%d V=%d tiles=%d "
-                     "lds=%zu/%zu | rows R=%d NT=%d V=%d blocks=%d maxn=%d lds=%zu\n",
-                     d.mx, d.my, d.nax, d.nay, d.N, h->fp32 ? 32 : 64, d.n_in, d.n_out, d.ngroups, h->cols.C,
-                     h->cols.NT, h->cols.V, h->cols.ntiles, h->cols.lds, h->cols.lds3, h->rows.R, h->rows.NT, h->rows.V,
-                     h->rows.nblocks, h->rows.maxn, h->rows.lds);
-    }
-    *out = h;
+    h->bytes = footprint_of(*h);
+    out = std::move(h);
     return SHHD_OK;
 }
 
-shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* in, size_t n) {
-    if (!h || !in) return set_err(h, SHHD_ERR_INVALID, "null argument");
+template <typename T>
+shhd_status upload_twiddles(Instance* h, std::string* err) {
+    std::vector<cx<T>> tx, ty;
+    make_twiddles<T>(h->cfg.nx, tx);
+    make_twiddles<T>(h->cfg.ny, ty);
+    SHHD_HIP(err, hipMalloc(&h->twx, tx.size() * sizeof(cx<T>)));
+    SHHD_HIP(err, hipMalloc(&h->twy, ty.size() * sizeof(cx<T>)));
+    SHHD_HIP(err, hipMemcpy(h->twx, tx.data(), tx.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    SHHD_HIP(err, hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    return SHHD_OK;
+}
+
+shhd_status allocate(Instance* h, std::string* err) {
+    const size_t nf = static_cast<size_t>(h->desc.nfields);
+    for (int b = 0; b < 3; ++b) {
+        SHHD_HIP(err, hipMalloc(&h->state[b], nf * h->field_elems * h->elem));
+        SHHD_HIP(err, hipMemset(h->state[b], 0, nf * h->field_elems * h->elem));
+        if (h->fp32) {
+            SHHD_HIP(err, hipMalloc(&h->k0[b], nf * sizeof(cx<double>)));
+            SHHD_HIP(err, hipMemset(h->k0[b], 0, nf * sizeof(cx<double>)));
+        }
+    }
+    SHHD_HIP(err, hipMalloc(&h->tbuf, std::max<size_t>(1, static_cast<size_t>(h->n_in)) * h->plane_elems * h->elem));
+    SHHD_HIP(err, hipMalloc(&h->ubuf, std::max<size_t>(1, static_cast<size_t>(h->n_out)) * h->plane_elems * h->elem));
+    return h->fp32 ? upload_twiddles<float>(h, err) : upload_twiddles<double>(h, err);
+}
+
+// Reference-layout state -> compact device layout (into buffer h->cur).
+shhd_status upload_state(Instance* h, const double* in, size_t n, hipStream_t stream, std::string* err) {
     const shhd_config& c = h->cfg;
     const size_t spec = ref_spec(c);
     const int nf = h->desc.nfields;
-    if (n != 2 * spec * static_cast<size_t>(nf)) return set_err(h, SHHD_ERR_INVALID, "state size mismatch");
+    if (n != 2 * spec * static_cast<size_t>(nf)) return set_err(err, SHHD_ERR_INVALID, "state size mismatch");
     const int hx = c.nx / 2 + 1;
     const int nay = c.ny_active;
     std::vector<cx<double>> compact(static_cast<size_t>(nf) * h->field_elems, czero<double>());
@@ -749,7 +649,7 @@ shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* in, size_t n) {
                 const double re = src[2 * i], im = src[2 * i + 1];
                 if (!ref_active(c, gx, gy)) {
                     if (re != 0.0 || im != 0.0)
-                        return set_err(h, SHHD_ERR_INVALID,
+                        return set_err(err, SHHD_ERR_INVALID,
                                        "state has nonzero inactive modes (the 3/2-rule mask must be applied)");
                     continue;
                 }
@@ -759,47 +659,49 @@ shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* in, size_t n) {
             }
         }
     }
-    SHHD_HIP(h, hipStreamSynchronize(h->stream));
-    if (h->fp32) {
-        std::vector<cx<float>> cf(compact.size());
-        for (size_t i = 0; i < compact.size(); ++i)
-            cf[i] = mk(static_cast<float>(compact[i].re), static_cast<float>(compact[i].im));
-        SHHD_HIP(h, hipMemcpy(h->state[0], cf.data(), cf.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
-        std::vector<cx<double>> k0(static_cast<size_t>(nf));
-        for (int f = 0; f < nf; ++f) k0[static_cast<size_t>(f)] = compact[static_cast<size_t>(f) * h->field_elems];
-        SHHD_HIP(h, hipMemcpy(h->k0[0], k0.data(), k0.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
-    } else {
-        SHHD_HIP(h, hipMemcpy(h->state[0], compact.data(), compact.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
+    SHHD_HIP(err, hipStreamSynchronize(stream));
+    // static fields live in buffer 0, the others in the current buffer
+    for (int f = 0; f < nf; ++f) {
+        const int b = h->is_static[f] ? 0 : h->cur;
+        const size_t off = static_cast<size_t>(f) * h->field_elems;
+        if (h->fp32) {
+            std::vector<cx<float>> cf(h->field_elems);
+            for (size_t i = 0; i < cf.size(); ++i)
+                cf[i] = mk(static_cast<float>(compact[off + i].re), static_cast<float>(compact[off + i].im));
+            SHHD_HIP(err, hipMemcpy(static_cast<cx<float>*>(h->state[b]) + off, cf.data(), cf.size() * sizeof(cx<float>),
+                                    hipMemcpyHostToDevice));
+            SHHD_HIP(err, hipMemcpy(h->k0[b] + f, &compact[off], sizeof(cx<double>), hipMemcpyHostToDevice));
+        } else {
+            SHHD_HIP(err, hipMemcpy(static_cast<cx<double>*>(h->state[b]) + off, compact.data() + off,
+                                    h->field_elems * sizeof(cx<double>), hipMemcpyHostToDevice));
+        }
     }
-    h->cur = 0;
-    h->state_set = true;
     // reference density rho_hat(0,0)/(Mx*My) (src/fcalculator_workspace.cc:41-66)
     h->desc.rho0 = compact[0].re / (static_cast<double>(c.nx) * static_cast<double>(c.ny));
     return SHHD_OK;
 }
 
-shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n) {
-    if (!h || !out) return set_err(h, SHHD_ERR_INVALID, "null argument");
+shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t stream, std::string* err) {
     const shhd_config& c = h->cfg;
     const size_t spec = ref_spec(c);
     const int nf = h->desc.nfields;
-    if (n != 2 * spec * static_cast<size_t>(nf)) return set_err(h, SHHD_ERR_INVALID, "state size mismatch");
-    SHHD_HIP(h, hipStreamSynchronize(h->stream));
+    if (n != 2 * spec * static_cast<size_t>(nf)) return set_err(err, SHHD_ERR_INVALID, "state size mismatch");
+    SHHD_HIP(err, hipStreamSynchronize(stream));
     std::vector<cx<double>> compact(static_cast<size_t>(nf) * h->field_elems);
     for (int f = 0; f < nf; ++f) {
         const int b = h->is_static[f] ? 0 : h->cur;
         const size_t off = static_cast<size_t>(f) * h->field_elems;
         if (h->fp32) {
             std::vector<cx<float>> cf(h->field_elems);
-            SHHD_HIP(h, hipMemcpy(cf.data(), static_cast<cx<float>*>(h->state[b]) + off, cf.size() * sizeof(cx<float>),
-                                  hipMemcpyDeviceToHost));
+            SHHD_HIP(err, hipMemcpy(cf.data(), static_cast<cx<float>*>(h->state[b]) + off, cf.size() * sizeof(cx<float>),
+                                    hipMemcpyDeviceToHost));
             for (size_t i = 0; i < cf.size(); ++i) compact[off + i] = mk<double>(cf[i].re, cf[i].im);
             cx<double> k0v;
-            SHHD_HIP(h, hipMemcpy(&k0v, h->k0[b] + f, sizeof(cx<double>), hipMemcpyDeviceToHost));
+            SHHD_HIP(err, hipMemcpy(&k0v, h->k0[b] + f, sizeof(cx<double>), hipMemcpyDeviceToHost));
             compact[off] = k0v;
         } else {
-            SHHD_HIP(h, hipMemcpy(compact.data() + off, static_cast<cx<double>*>(h->state[b]) + off,
-                                  h->field_elems * sizeof(cx<double>), hipMemcpyDeviceToHost));
+            SHHD_HIP(err, hipMemcpy(compact.data() + off, static_cast<cx<double>*>(h->state[b]) + off,
+                                    h->field_elems * sizeof(cx<double>), hipMemcpyDeviceToHost));
         }
     }
     std::memset(out, 0, sizeof(double) * n);
@@ -822,146 +724,440 @@ shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n) {
     return SHHD_OK;
 }
 
-shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0) {
-    (void)t0;   // stage times are cache keys only in the reference
-    if (!h) return set_err(h, SHHD_ERR_INVALID, "null handle");
-    if (!h->state_set) return set_err(h, SHHD_ERR_INVALID, "shhd_gpu_set_state must be called before stepping");
-    if (nsteps < 0) return set_err(h, SHHD_ERR_INVALID, "nsteps must be nonnegative");
-    if (nsteps == 0) return SHHD_OK;
-    if (h->rho0_needed && h->desc.rho0 == 0.0) return set_err(h, SHHD_ERR_RUNTIME, "reference density must be nonzero.");
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->dflag);
-    if (!h->static_fields.empty()) {
-        const int ns = static_cast<int>(h->static_fields.size());
-        if (h->fp32)
-            hipLaunchKernelGGL(k_enforce_static<float>, dim3(ns), dim3(256), 0, h->stream,
-                               static_cast<cx<float>*>(h->state[0]), h->desc.nay, h->field_elems, h->dstatic, ns, h->k0[0]);
-        else
-            hipLaunchKernelGGL(k_enforce_static<double>, dim3(ns), dim3(256), 0, h->stream,
-                               static_cast<cx<double>*>(h->state[0]), h->desc.nay, h->field_elems, h->dstatic, ns,
-                               static_cast<cx<double>*>(nullptr));
-        SHHD_HIP(h, hipGetLastError());
+// ---------------------------------------------------------------------------
+// engine: instances grouped into launch classes
+// ---------------------------------------------------------------------------
+struct ClassKey {
+    bool fp32;
+    int stages, cNT, cV, acc, rNT, rV, maxn;
+    bool operator<(const ClassKey& o) const {
+        return std::tie(fp32, stages, cNT, cV, acc, rNT, rV, maxn) <
+               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.acc, o.rNT, o.rV, o.maxn);
     }
-    static constexpr double S2 = 1.41421356237309504880;
-    static constexpr double S3 = 1.73205080756887729352;
-    static constexpr double B1 = (2.0 * S2 + S3) / 5.0;
-    static constexpr double B2 = (-4.0 * S2 + 3.0 * S3) / 5.0;
-    static constexpr double B3 = (S2 - 2.0 * S3) / 10.0;
-    for (int s = 0; s < nsteps; ++s) {
-        StageParams sp{};
-        sp.step_lo = static_cast<uint32_t>(h->step_index);
-        sp.step_hi = static_cast<uint32_t>(h->step_index >> 32);
-        const int x = h->cur, y = (h->cur + 1) % 3, z = (h->cur + 2) % 3;
+};
+
+struct LaunchClass {
+    ClassKey key{};
+    std::vector<int> members;        // slots
+    void* d_inst = nullptr;          // InstDev<T>[n]
+    void* d_ptrs = nullptr;          // StagePtrs<T>[3 * stages][n]
+    int* d_map = nullptr;            // block maps of K1, K2, K3
+    void* d_static = nullptr;        // StaticEntry<T>[nstatic]
+    int nstatic = 0;
+    int nblk[3] = {0, 0, 0};
+    BlockMap map[3] = {};
+    size_t lds[3] = {0, 0, 0};
+
+    void release() {
+        if (d_inst) (void)hipFree(d_inst);
+        if (d_ptrs) (void)hipFree(d_ptrs);
+        if (d_map) (void)hipFree(d_map);
+        if (d_static) (void)hipFree(d_static);
+        d_inst = d_ptrs = d_static = nullptr;
+        d_map = nullptr;
+    }
+};
+
+struct Engine {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool batch_shape = false;     // plan small grids onto the shared 256-lane shape
+    bool use_graph = false;
+    size_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<Instance>> slots;
+    uint64_t* d_steps = nullptr;
+    int* d_flags = nullptr;
+    std::vector<LaunchClass> classes;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    bool timing = false;
+    double time_ms[kClasses] = {0, 0, 0};
+    long long launches[kClasses] = {0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+
+    ~Engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        for (auto& c : classes) c.release();
+        slots.clear();
+        if (d_steps) (void)hipFree(d_steps);
+        if (d_flags) (void)hipFree(d_flags);
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        for (auto& u : ev_used) {
+            (void)hipEventDestroy(u.second.first);
+            (void)hipEventDestroy(u.second.second);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+
+    shhd_status init(size_t budget_bytes, bool batch, bool graphs) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            return set_err(&err, SHHD_ERR_HIP, "no HIP device available");
+        SHHD_HIP(&err, hipGetDevice(&device));
+        SHHD_HIP(&err, hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        SHHD_HIP(&err, hipMalloc(&d_steps, kMaxSlots * sizeof(uint64_t)));
+        SHHD_HIP(&err, hipMalloc(&d_flags, kMaxSlots * sizeof(int)));
+        SHHD_HIP(&err, hipMemset(d_steps, 0, kMaxSlots * sizeof(uint64_t)));
+        SHHD_HIP(&err, hipMemset(d_flags, 0, kMaxSlots * sizeof(int)));
+        budget = budget_bytes;
+        batch_shape = batch;
+        use_graph = graphs;
+        slots.resize(kMaxSlots);
+        return SHHD_OK;
+    }
+
+    int active_count() const {
+        int n = 0;
+        for (auto& s : slots) n += s ? 1 : 0;
+        return n;
+    }
+
+    shhd_status admit(const shhd_config& cfg, const double* state, size_t n, int* slot_out) {
+        std::unique_ptr<Instance> h;
+        shhd_status st = prepare(cfg, batch_shape, h);
+        if (st != SHHD_OK) return set_err(&err, st, g_create_error);
+        if (budget && used + h->bytes > budget)
+            return set_err(&err, SHHD_ERR_FULL, "device budget exhausted: instance needs " + std::to_string(h->bytes) +
+                                                     " bytes, " + std::to_string(budget - used) + " left");
+        int slot = -1;
+        for (int i = 0; i < kMaxSlots; ++i)
+            if (!slots[static_cast<size_t>(i)]) { slot = i; break; }
+        if (slot < 0) return set_err(&err, SHHD_ERR_FULL, "no free instance slot");
+        st = allocate(h.get(), &err);
+        if (st != SHHD_OK) return st;
+        if (state) {
+            st = upload_state(h.get(), state, n, stream, &err);
+            if (st != SHHD_OK) return st;
+        }
+        const uint64_t zero = 0;
+        SHHD_HIP(&err, hipMemcpy(d_steps + slot, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        used += h->bytes;
+        slots[static_cast<size_t>(slot)] = std::move(h);
+        dirty = true;
+        *slot_out = slot;
+        return SHHD_OK;
+    }
+
+    shhd_status retire(int slot) {
+        if (slot < 0 || slot >= kMaxSlots || !slots[static_cast<size_t>(slot)])
+            return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        used -= slots[static_cast<size_t>(slot)]->bytes;
+        slots[static_cast<size_t>(slot)].reset();
+        dirty = true;
+        return SHHD_OK;
+    }
+
+    Instance* at(int slot) {
+        if (slot < 0 || slot >= kMaxSlots) return nullptr;
+        return slots[static_cast<size_t>(slot)].get();
+    }
+
+    template <typename T>
+    StagePtrs<T> stage_ptrs(const Instance& h, int next, int cur, int base) const {
+        StagePtrs<T> p{};
+        for (int f = 0; f < h.desc.nfields; ++f) {
+            auto at = [&](int b) {
+                const int bb = h.is_static[f] ? 0 : b;
+                return static_cast<cx<T>*>(h.state[bb]) + static_cast<size_t>(f) * h.field_elems;
+            };
+            p.cur[f] = at(cur);
+            p.base[f] = at(base);
+            p.next[f] = at(next);
+        }
+        if (h.fp32) {
+            p.cur_k0 = h.k0[cur];
+            p.base_k0 = h.k0[base];
+            p.next_k0 = h.k0[next];
+        }
+        return p;
+    }
+
+    template <typename T>
+    shhd_status build_class_tables(LaunchClass& lc) {
+        const int n = static_cast<int>(lc.members.size());
+        const int stages = lc.key.stages;
+        std::vector<InstDev<T>> inst(static_cast<size_t>(n));
+        std::vector<StagePtrs<T>> ptrs(static_cast<size_t>(3 * stages * n));
+        std::vector<int> maps[3];
+        std::vector<int> firsts[3];
+        std::vector<StaticEntry<T>> statics;
+        for (int k = 0; k < 3; ++k) firsts[k].resize(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            Instance& h = *slots[static_cast<size_t>(lc.members[static_cast<size_t>(i)])];
+            InstDev<T>& I = inst[static_cast<size_t>(i)];
+            I.d = h.desc;
+            I.twx = static_cast<const cx<T>*>(h.twx);
+            I.twy = static_cast<const cx<T>*>(h.twy);
+            I.tbuf = static_cast<cx<T>*>(h.tbuf);
+            I.ubuf = static_cast<cx<T>*>(h.ubuf);
+            I.flag = d_flags + lc.members[static_cast<size_t>(i)];
+            I.step = d_steps + lc.members[static_cast<size_t>(i)];
+            I.C = h.cols.C;
+            I.R = h.rows.R;
+            I.ntiles = h.cols.ntiles;
+            // stage pointers for the k-th step after this build, rotating the
+            // three buffers without copies: SRK3 stages (next, cur, base) =
+            // (y, x, x), (z, y, x), (y, z, x); Euler (y, x, x); then cur <- y
+            for (int k = 0; k < 3; ++k) {
+                const int x = (h.cur + k) % 3, y = (h.cur + k + 1) % 3, z = (h.cur + k + 2) % 3;
+                const int tab[3][3] = {{y, x, x}, {z, y, x}, {y, z, x}};
+                for (int s = 0; s < stages; ++s)
+                    ptrs[static_cast<size_t>((k * stages + s) * n + i)] = stage_ptrs<T>(h, tab[s][0], tab[s][1], tab[s][2]);
+            }
+            const int nb[3] = {h.desc.n_in > 0 ? h.cols.ntiles * h.desc.n_in : 0,
+                               (h.desc.n_in > 0 || h.desc.n_out > 0) ? h.rows.nblocks : 0,
+                               h.cols.ntiles * h.desc.ngroups};
+            for (int kk = 0; kk < 3; ++kk) {
+                firsts[kk][static_cast<size_t>(i)] = static_cast<int>(maps[kk].size());
+                for (int b = 0; b < nb[kk]; ++b) maps[kk].push_back(i);
+            }
+            for (int f : h.static_fields)
+                statics.push_back({static_cast<cx<T>*>(h.state[0]) + static_cast<size_t>(f) * h.field_elems,
+                                   h.fp32 ? h.k0[0] + f : nullptr, h.desc.nay});
+            lc.lds[0] = std::max(lc.lds[0], h.cols.lds);
+            lc.lds[1] = std::max(lc.lds[1], h.rows.lds);
+            lc.lds[2] = std::max(lc.lds[2], h.cols.lds3);
+        }
+        SHHD_HIP(&err, hipMalloc(&lc.d_inst, inst.size() * sizeof(InstDev<T>)));
+        SHHD_HIP(&err, hipMemcpy(lc.d_inst, inst.data(), inst.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
+        SHHD_HIP(&err, hipMalloc(&lc.d_ptrs, ptrs.size() * sizeof(StagePtrs<T>)));
+        SHHD_HIP(&err, hipMemcpy(lc.d_ptrs, ptrs.data(), ptrs.size() * sizeof(StagePtrs<T>), hipMemcpyHostToDevice));
+        size_t total = 0;
+        for (int k = 0; k < 3; ++k) total += maps[k].size() + firsts[k].size();
+        SHHD_HIP(&err, hipMalloc(&lc.d_map, std::max<size_t>(1, total) * sizeof(int)));
+        size_t off = 0;
+        for (int k = 0; k < 3; ++k) {
+            lc.nblk[k] = static_cast<int>(maps[k].size());
+            lc.map[k].inst = lc.d_map + off;
+            if (!maps[k].empty())
+                SHHD_HIP(&err, hipMemcpy(lc.d_map + off, maps[k].data(), maps[k].size() * sizeof(int), hipMemcpyHostToDevice));
+            off += maps[k].size();
+            lc.map[k].first = lc.d_map + off;
+            SHHD_HIP(&err, hipMemcpy(lc.d_map + off, firsts[k].data(), firsts[k].size() * sizeof(int), hipMemcpyHostToDevice));
+            off += firsts[k].size();
+        }
+        lc.nstatic = static_cast<int>(statics.size());
+        if (lc.nstatic) {
+            SHHD_HIP(&err, hipMalloc(&lc.d_static, statics.size() * sizeof(StaticEntry<T>)));
+            SHHD_HIP(&err, hipMemcpy(lc.d_static, statics.data(), statics.size() * sizeof(StaticEntry<T>), hipMemcpyHostToDevice));
+        }
+        const ClassKey& K = lc.key;
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV)),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn)),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.cNT, K.cV, K.acc)),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
+        return SHHD_OK;
+    }
+
+    shhd_status rebuild() {
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        drop_graph();
+        for (auto& c : classes) c.release();
+        classes.clear();
+        std::map<ClassKey, std::vector<int>> groups;
+        for (int s = 0; s < kMaxSlots; ++s) {
+            const Instance* h = slots[static_cast<size_t>(s)].get();
+            if (!h) continue;
+            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols.acc, h->rows.NT, h->rows.V, h->rows.maxn};
+            groups[k].push_back(s);
+        }
+        for (auto& g : groups) {
+            LaunchClass lc;
+            lc.key = g.first;
+            lc.members = g.second;
+            classes.push_back(lc);
+        }
+        for (auto& lc : classes) {
+            shhd_status st = lc.key.fp32 ? build_class_tables<float>(lc) : build_class_tables<double>(lc);
+            if (st != SHHD_OK) return st;
+        }
+        dirty = false;
+        return SHHD_OK;
+    }
+
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+
+    template <typename F>
+    shhd_status timed(int cls, bool capturing, F&& launch) {
+        const bool t = timing && !capturing;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (t) {
+            a = take_event();
+            b = take_event();
+            SHHD_HIP(&err, hipEventRecord(a, stream));
+        }
+        launch();
+        SHHD_HIP(&err, hipGetLastError());
+        if (t) {
+            SHHD_HIP(&err, hipEventRecord(b, stream));
+            ev_used.push_back({cls, {a, b}});
+        }
+        return SHHD_OK;
+    }
+
+    template <typename T>
+    shhd_status launch_stage(LaunchClass& lc, int k, int s, const StageParams& sp, bool capturing) {
+        const ClassKey& K = lc.key;
+        const int n = static_cast<int>(lc.members.size());
+        const InstDev<T>* inst = static_cast<const InstDev<T>*>(lc.d_inst);
+        const StagePtrs<T>* ptrs = static_cast<const StagePtrs<T>*>(lc.d_ptrs) + static_cast<size_t>(k * K.stages + s) * n;
         shhd_status st = SHHD_OK;
-        auto stage = [&](int next, int cur, int base, double beta, double wb, double wc, int use_b) {
-            sp.beta = beta;
-            sp.wb = wb;
-            sp.wc = wc;
-            sp.use_b = use_b;
-            return h->fp32 ? run_stage<float>(h, next, cur, base, sp) : run_stage<double>(h, next, cur, base, sp);
-        };
-        if (h->stages == 3) {
-            // time_integrator_srk3_*.h: (w_base, w_current) = (0,1), (3/4,1/4), (1/3,2/3)
-            if ((st = stage(y, x, x, B1, 0.0, 1.0, 1)) != SHHD_OK) return st;
-            if ((st = stage(z, y, x, B2, 3.0 / 4.0, 1.0 / 4.0, 1)) != SHHD_OK) return st;
-            if ((st = stage(y, z, x, B3, 1.0 / 3.0, 2.0 / 3.0, 1)) != SHHD_OK) return st;
-            h->cur = y;
-        } else {
-            if ((st = stage(y, x, x, 0.0, 0.0, 1.0, 0)) != SHHD_OK) return st;
-            h->cur = y;
+        if (lc.nblk[0] > 0) {
+            K1Fn<T> k1 = pick_k1<T>(K.cNT, K.cV);
+            st = timed(0, capturing, [&] {
+                hipLaunchKernelGGL(k1, dim3(lc.nblk[0]), dim3(K.cNT), lc.lds[0], stream, inst, ptrs, lc.map[0]);
+            });
+            if (st != SHHD_OK) return st;
         }
-        h->step_index++;
-    }
-    int flag = 0;
-    SHHD_HIP(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    SHHD_HIP(h, hipStreamSynchronize(h->stream));
-    if (h->timing) {
-        for (auto& u : h->ev_used) {
-            float ms = 0.f;
-            SHHD_HIP(h, hipEventElapsedTime(&ms, u.second.first, u.second.second));
-            h->time_ms[u.first] += ms;
-            h->launches[u.first] += 1;
-            h->ev_pool.push_back(u.second.first);
-            h->ev_pool.push_back(u.second.second);
+        if (lc.nblk[1] > 0) {
+            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn);
+            st = timed(1, capturing, [&] {
+                hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], stream, inst, lc.map[1]);
+            });
+            if (st != SHHD_OK) return st;
         }
-        h->ev_used.clear();
+        K3Fn<T> k3 = pick_k3<T>(K.cNT, K.cV, K.acc);
+        return timed(2, capturing, [&] {
+            hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.cNT), lc.lds[2], stream, inst, ptrs, sp, lc.map[2]);
+        });
     }
-    if (flag) return set_err(h, SHHD_ERR_ZERO_DENSITY, "compressible velocity encountered zero density.");
-    return SHHD_OK;
-}
 
-shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parameter, int momentum) {
-    if (!h) return set_err(h, SHHD_ERR_INVALID, "null handle");
-    if (density || order_parameter || momentum)
-        return set_err(h, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
-    return SHHD_OK;
-}
-
-shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n) {
-    (void)out;
-    (void)n;
-    return set_err(h, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
-}
-
-void shhd_gpu_destroy(shhd_gpu* h) {
-    if (!h) return;
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (int b = 0; b < 3; ++b) {
-        if (h->state[b]) (void)hipFree(h->state[b]);
-        if (h->k0[b]) (void)hipFree(h->k0[b]);
+    // one step of every instance, k = steps since the tables were built (mod 3)
+    shhd_status one_step(int k, bool capturing) {
+        static constexpr double S2 = 1.41421356237309504880;
+        static constexpr double S3 = 1.73205080756887729352;
+        static constexpr double B[3] = {(2.0 * S2 + S3) / 5.0, (-4.0 * S2 + 3.0 * S3) / 5.0, (S2 - 2.0 * S3) / 10.0};
+        // time_integrator_srk3_*.h: (w_base, w_current) = (0,1), (3/4,1/4), (1/3,2/3)
+        static constexpr double WB[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+        static constexpr double WC[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+        for (auto& lc : classes) {
+            if (!lc.nstatic) continue;
+            if (lc.key.fp32)
+                hipLaunchKernelGGL(k_enforce_static<float>, dim3(lc.nstatic), dim3(256), 0, stream,
+                                   static_cast<const StaticEntry<float>*>(lc.d_static));
+            else
+                hipLaunchKernelGGL(k_enforce_static<double>, dim3(lc.nstatic), dim3(256), 0, stream,
+                                   static_cast<const StaticEntry<double>*>(lc.d_static));
+            SHHD_HIP(&err, hipGetLastError());
+        }
+        for (int s = 0; s < 3; ++s) {
+            for (auto& lc : classes) {
+                if (s >= lc.key.stages) continue;
+                StageParams sp{};
+                if (lc.key.stages == 3) {
+                    sp.beta = B[s];
+                    sp.wb = WB[s];
+                    sp.wc = WC[s];
+                    sp.use_b = 1;
+                } else {
+                    sp.beta = 0.0;
+                    sp.wb = 0.0;
+                    sp.wc = 1.0;
+                    sp.use_b = 0;
+                }
+                shhd_status st = lc.key.fp32 ? launch_stage<float>(lc, k, s, sp, capturing)
+                                             : launch_stage<double>(lc, k, s, sp, capturing);
+                if (st != SHHD_OK) return st;
+            }
+        }
+        hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, kMaxSlots);
+        SHHD_HIP(&err, hipGetLastError());
+        return SHHD_OK;
     }
-    if (h->tbuf) (void)hipFree(h->tbuf);
-    if (h->ubuf) (void)hipFree(h->ubuf);
-    if (h->twx) (void)hipFree(h->twx);
-    if (h->twy) (void)hipFree(h->twy);
-    if (h->dflag) (void)hipFree(h->dflag);
-    if (h->dstatic) (void)hipFree(h->dstatic);
-    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
-    for (auto& u : h->ev_used) {
-        (void)hipEventDestroy(u.second.first);
-        (void)hipEventDestroy(u.second.second);
+
+    shhd_status step(int nsteps) {
+        if (nsteps < 0) return set_err(&err, SHHD_ERR_INVALID, "nsteps must be nonnegative");
+        if (nsteps == 0 || active_count() == 0) return SHHD_OK;
+        for (auto& sp : slots) {
+            if (sp && sp->rho0_needed && sp->desc.rho0 == 0.0)
+                return set_err(&err, SHHD_ERR_RUNTIME, "reference density must be nonzero.");
+        }
+        if (dirty) {
+            shhd_status st = rebuild();
+            if (st != SHHD_OK) return st;
+        }
+        hipLaunchKernelGGL(k_clear_flags, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_flags, kMaxSlots);
+        SHHD_HIP(&err, hipGetLastError());
+        int done = 0;
+        if (use_graph && !timing && nsteps >= 3) {
+            if (!graph_exec) {
+                SHHD_HIP(&err, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                shhd_status st = SHHD_OK;
+                for (int k = 0; k < 3 && st == SHHD_OK; ++k) st = one_step(k, true);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(stream, &g);
+                if (st != SHHD_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return st;
+                }
+                if (ec != hipSuccess) return set_err(&err, SHHD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+                graph = g;
+                SHHD_HIP(&err, hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            }
+            for (; done + 3 <= nsteps; done += 3) SHHD_HIP(&err, hipGraphLaunch(graph_exec, stream));
+        }
+        for (int k = 0; done < nsteps; ++done, ++k) {
+            shhd_status st = one_step(k % 3, false);
+            if (st != SHHD_OK) return st;
+        }
+        for (auto& sp : slots) {
+            if (!sp) continue;
+            sp->cur = (sp->cur + nsteps) % 3;
+            sp->step_index += static_cast<uint64_t>(nsteps);
+        }
+        if (nsteps % 3 != 0) dirty = true;   // stage-pointer tables are built for the current rotation
+        std::vector<int> flags(kMaxSlots);
+        SHHD_HIP(&err, hipMemcpyAsync(flags.data(), d_flags, kMaxSlots * sizeof(int), hipMemcpyDeviceToHost, stream));
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        if (timing) {
+            for (auto& u : ev_used) {
+                float ms = 0.f;
+                SHHD_HIP(&err, hipEventElapsedTime(&ms, u.second.first, u.second.second));
+                time_ms[u.first] += ms;
+                launches[u.first] += 1;
+                ev_pool.push_back(u.second.first);
+                ev_pool.push_back(u.second.second);
+            }
+            ev_used.clear();
+        }
+        for (int s = 0; s < kMaxSlots; ++s) {
+            if (slots[static_cast<size_t>(s)] && flags[static_cast<size_t>(s)])
+                return set_err(&err, SHHD_ERR_ZERO_DENSITY, "compressible velocity encountered zero density.");
+        }
+        return SHHD_OK;
     }
-    if (h->stream) (void)hipStreamDestroy(h->stream);
-    delete h;
-}
+};
 
-const char* shhd_gpu_last_error(const shhd_gpu* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
-
-shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info) {
-    if (!h || !info) return SHHD_ERR_INVALID;
-    info->num_fields = h->desc.nfields;
-    info->num_inverse_fields = h->n_in;
-    info->num_forward_fields = h->n_out;
-    info->num_updated_fields = h->n_updated;
-    info->stages_per_step = h->stages;
-    info->state_doubles = 2 * ref_spec(h->cfg) * static_cast<size_t>(h->desc.nfields);
-    info->device_bytes = h->device_bytes;
-    return SHHD_OK;
-}
-
-shhd_status shhd_gpu_set_kernel_timing(shhd_gpu* h, int enable) {
-    if (!h) return SHHD_ERR_INVALID;
-    h->timing = enable != 0;
-    for (int i = 0; i < kClasses; ++i) {
-        h->time_ms[i] = 0.0;
-        h->launches[i] = 0;
-    }
-    return SHHD_OK;
-}
-
-shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long* launches, int nclasses) {
-    if (!h || !total_ms || !launches) return SHHD_ERR_INVALID;
-    for (int i = 0; i < nclasses && i < kClasses; ++i) {
-        total_ms[i] = h->time_ms[i];
-        launches[i] = h->launches[i];
-    }
-    return SHHD_OK;
-}
-
-shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int nclasses) {
-    if (!h || !bytes) return SHHD_ERR_INVALID;
-    const Desc& d = h->desc;
-    const double fb = static_cast<double>(h->field_elems) * static_cast<double>(h->elem);
-    const double pb = static_cast<double>(h->plane_elems) * static_cast<double>(h->elem);
+double kernel_bytes_of(const Instance& h, int cls) {
+    const Desc& d = h.desc;
+    const double fb = static_cast<double>(h.field_elems) * static_cast<double>(h.elem);
+    const double pb = static_cast<double>(h.plane_elems) * static_cast<double>(h.elem);
+    if (cls == 0) return d.n_in * (fb + pb);
+    if (cls == 1) return (d.n_in + d.n_out) * pb;
     double k3 = d.n_out * pb;
     for (int g = 0; g < d.ngroups; ++g) {
         const Group& G = d.groups[g];
@@ -974,9 +1170,183 @@ shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int ncla
             k3 += 6.0 * fb + (d.pressure_coefficient != 0.0 ? fb : 0.0);
         }
     }
-    const double vals[kClasses] = {d.n_in * (fb + pb), (d.n_in + d.n_out) * pb, k3};
-    for (int i = 0; i < nclasses && i < kClasses; ++i) bytes[i] = vals[i];
+    return k3;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI: single simulation
+// ---------------------------------------------------------------------------
+struct shhd_gpu {
+    Engine eng;
+    int slot = -1;
+    bool state_set = false;
+    Instance* inst() { return eng.at(slot); }
+};
+
+struct shhd_batch {
+    Engine eng;
+};
+
+extern "C" {
+
+shhd_status shhd_gpu_create(const shhd_config* config, shhd_gpu** out) {
+    if (!config || !out) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<Instance> probe;
+    shhd_status st = prepare(*config, false, probe);   // configuration errors before any device work
+    if (st != SHHD_OK) return st;
+    auto h = std::make_unique<shhd_gpu>();
+    const char* g = std::getenv("SHHD_USE_GRAPH");
+    st = h->eng.init(0, false, g && std::atoi(g) != 0);
+    if (st != SHHD_OK) return set_err(nullptr, st, h->eng.err);
+    st = h->eng.admit(*config, nullptr, 0, &h->slot);
+    if (st != SHHD_OK) return set_err(nullptr, st, h->eng.err);
+    if (std::getenv("SHHD_VERBOSE")) {
+        const Instance& I = *h->inst();
+        const Desc& d = I.desc;
+        std::fprintf(stderr,
+                     "[shhd] Mx=%d My=%d Na=%dx%d N=%d fp%d F_in=%d F_out=%d groups=%d | cols C=%d NT=%d V=%d acc=%d "
+                     "tiles=%d lds=%zu/%zu | rows R=%d NT=%d V=%d blocks=%d maxn=%d lds=%zu\n",
+                     d.mx, d.my, d.nax, d.nay, d.N, I.fp32 ? 32 : 64, d.n_in, d.n_out, d.ngroups, I.cols.C, I.cols.NT,
+                     I.cols.V, I.cols.acc, I.cols.ntiles, I.cols.lds, I.cols.lds3, I.rows.R, I.rows.NT, I.rows.V,
+                     I.rows.nblocks, I.rows.maxn, I.rows.lds);
+    }
+    *out = h.release();
     return SHHD_OK;
 }
+
+shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* in, size_t n) {
+    if (!h || !in) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    Instance* I = h->inst();
+    shhd_status st = upload_state(I, in, n, h->eng.stream, &h->eng.err);
+    if (st != SHHD_OK) return st;
+    h->state_set = true;
+    h->eng.dirty = true;   // rho0 of the descriptor may have changed
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n) {
+    if (!h || !out) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return download_state(h->inst(), out, n, h->eng.stream, &h->eng.err);
+}
+
+shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0) {
+    (void)t0;   // stage times are cache keys only in the reference
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    if (!h->state_set) return set_err(&h->eng.err, SHHD_ERR_INVALID, "shhd_gpu_set_state must be called before stepping");
+    return h->eng.step(nsteps);
+}
+
+shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parameter, int momentum) {
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    if (density || order_parameter || momentum)
+        return set_err(&h->eng.err, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n) {
+    (void)out;
+    (void)n;
+    return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
+}
+
+void shhd_gpu_destroy(shhd_gpu* h) { delete h; }
+
+const char* shhd_gpu_last_error(const shhd_gpu* h) { return h ? h->eng.err.c_str() : g_create_error.c_str(); }
+
+shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info) {
+    if (!h || !info) return SHHD_ERR_INVALID;
+    const Instance* I = const_cast<shhd_gpu*>(h)->inst();
+    info->num_fields = I->desc.nfields;
+    info->num_inverse_fields = I->n_in;
+    info->num_forward_fields = I->n_out;
+    info->num_updated_fields = I->n_updated;
+    info->stages_per_step = I->stages;
+    info->state_doubles = 2 * ref_spec(I->cfg) * static_cast<size_t>(I->desc.nfields);
+    info->device_bytes = I->bytes;
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_set_kernel_timing(shhd_gpu* h, int enable) {
+    if (!h) return SHHD_ERR_INVALID;
+    h->eng.timing = enable != 0;
+    for (int i = 0; i < kClasses; ++i) {
+        h->eng.time_ms[i] = 0.0;
+        h->eng.launches[i] = 0;
+    }
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long* launches, int nclasses) {
+    if (!h || !total_ms || !launches) return SHHD_ERR_INVALID;
+    for (int i = 0; i < nclasses && i < kClasses; ++i) {
+        total_ms[i] = h->eng.time_ms[i];
+        launches[i] = h->eng.launches[i];
+    }
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int nclasses) {
+    if (!h || !bytes) return SHHD_ERR_INVALID;
+    const Instance* I = const_cast<shhd_gpu*>(h)->inst();
+    for (int i = 0; i < nclasses && i < kClasses; ++i) bytes[i] = kernel_bytes_of(*I, i);
+    return SHHD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI: batched instances
+// ---------------------------------------------------------------------------
+shhd_status shhd_batch_create(size_t device_budget_bytes, int use_graphs, shhd_batch** out) {
+    if (!out) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
+    auto b = std::make_unique<shhd_batch>();
+    shhd_status st = b->eng.init(device_budget_bytes, true, use_graphs != 0);
+    if (st != SHHD_OK) return set_err(nullptr, st, b->eng.err);
+    *out = b.release();
+    return SHHD_OK;
+}
+
+shhd_status shhd_batch_footprint(const shhd_config* config, size_t* bytes) {
+    if (!config || !bytes) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
+    std::unique_ptr<Instance> h;
+    shhd_status st = prepare(*config, true, h);
+    if (st != SHHD_OK) return st;
+    *bytes = h->bytes;
+    return SHHD_OK;
+}
+
+shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const double* state, size_t n, int* slot) {
+    if (!b || !config || !state || !slot) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return b->eng.admit(*config, state, n, slot);
+}
+
+shhd_status shhd_batch_step(shhd_batch* b, int nsteps) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.step(nsteps);
+}
+
+shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n) {
+    if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    const Instance* I = b->eng.at(slot);
+    if (!I) return set_err(&b->eng.err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+    return download_state(I, out, n, b->eng.stream, &b->eng.err);
+}
+
+shhd_status shhd_batch_retire(shhd_batch* b, int slot) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.retire(slot);
+}
+
+shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes) {
+    if (!b) return SHHD_ERR_INVALID;
+    if (active) *active = b->eng.active_count();
+    if (used_bytes) *used_bytes = b->eng.used;
+    return SHHD_OK;
+}
+
+void shhd_batch_destroy(shhd_batch* b) { delete b; }
+
+const char* shhd_batch_last_error(const shhd_batch* b) { return b ? b->eng.err.c_str() : g_create_error.c_str(); }
 
 }  // extern "C"

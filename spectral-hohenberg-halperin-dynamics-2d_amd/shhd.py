@@ -30,7 +30,8 @@ INTEGRATOR_EULER, INTEGRATOR_SRK3 = 0, 1
 FREE_ENERGY_NULL, FREE_ENERGY_QUADRATIC, FREE_ENERGY_PHI4 = 0, 1, 2
 
 STATUS = {0: "SHHD_OK", 1: "SHHD_ERR_INVALID", 2: "SHHD_ERR_UNSUPPORTED", 3: "SHHD_ERR_HIP",
-          4: "SHHD_ERR_ZERO_DENSITY", 5: "SHHD_ERR_RUNTIME"}
+          4: "SHHD_ERR_ZERO_DENSITY", 5: "SHHD_ERR_RUNTIME", 6: "SHHD_ERR_FULL"}
+ERR_FULL = 6
 
 
 class ShhdSineForce(ctypes.Structure):
@@ -247,6 +248,25 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_gpu_get_kernel_timing.restype = ctypes.c_int
     lib.shhd_gpu_get_kernel_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.shhd_gpu_get_kernel_bytes.restype = ctypes.c_int
+    lib.shhd_batch_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(P)]
+    lib.shhd_batch_create.restype = ctypes.c_int
+    lib.shhd_batch_footprint.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.POINTER(ctypes.c_size_t)]
+    lib.shhd_batch_footprint.restype = ctypes.c_int
+    lib.shhd_batch_admit.argtypes = [P, ctypes.POINTER(ShhdConfig), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_int)]
+    lib.shhd_batch_admit.restype = ctypes.c_int
+    lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
+    lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_batch_get_state.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    lib.shhd_batch_get_state.restype = ctypes.c_int
+    lib.shhd_batch_retire.argtypes = [P, ctypes.c_int]
+    lib.shhd_batch_retire.restype = ctypes.c_int
+    lib.shhd_batch_usage.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+    lib.shhd_batch_usage.restype = ctypes.c_int
+    lib.shhd_batch_destroy.argtypes = [P]
+    lib.shhd_batch_destroy.restype = None
+    lib.shhd_batch_last_error.argtypes = [P]
+    lib.shhd_batch_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -334,3 +354,80 @@ class GpuFluidSolver:
         b = (ctypes.c_double * 3)()
         self._check(self.lib.shhd_gpu_get_kernel_bytes(self.h, b, 3))
         return {k: b[i] for i, k in enumerate(self.KERNEL_CLASSES)}
+
+
+class Batch:
+    """Many resident simulations advanced together (include/shhd_gpu.h,
+    shhd_batch_*).  Each admitted instance computes exactly what it would
+    compute alone."""
+
+    def __init__(self, budget_bytes: int = 0, use_graphs: bool = True):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        st = self.lib.shhd_batch_create(int(budget_bytes), int(bool(use_graphs)), ctypes.byref(h))
+        if st != 0:
+            raise ShhdError(st, self.lib.shhd_batch_last_error(None).decode())
+        self.h = h
+        self.cfgs = {}
+
+    def _check(self, st):
+        if st != 0:
+            raise ShhdError(st, self.lib.shhd_batch_last_error(self.h).decode())
+
+    @staticmethod
+    def footprint(cfg: SolverConfig) -> int:
+        lib = load_library()
+        c = cfg.to_c()
+        n = ctypes.c_size_t()
+        st = lib.shhd_batch_footprint(ctypes.byref(c), ctypes.byref(n))
+        if st != 0:
+            raise ShhdError(st, lib.shhd_batch_last_error(None).decode())
+        return n.value
+
+    def admit(self, cfg: SolverConfig, state: np.ndarray) -> int:
+        c = cfg.to_c()
+        s = np.ascontiguousarray(state, dtype=np.complex128)
+        if s.shape != cfg.state_shape():
+            raise ValueError(f"state shape {s.shape} != {cfg.state_shape()}")
+        flat = s.view(np.float64).reshape(-1)
+        slot = ctypes.c_int()
+        self._check(self.lib.shhd_batch_admit(self.h, ctypes.byref(c), _dptr(flat), flat.size, ctypes.byref(slot)))
+        self.cfgs[slot.value] = cfg
+        return slot.value
+
+    def step(self, nsteps: int):
+        self._check(self.lib.shhd_batch_step(self.h, int(nsteps)))
+
+    def get_state(self, slot: int) -> np.ndarray:
+        cfg = self.cfgs[slot]
+        out = np.zeros(cfg.state_shape(), dtype=np.complex128)
+        flat = out.view(np.float64).reshape(-1)
+        self._check(self.lib.shhd_batch_get_state(self.h, int(slot), _dptr(flat), flat.size))
+        return out
+
+    def retire(self, slot: int):
+        self._check(self.lib.shhd_batch_retire(self.h, int(slot)))
+        self.cfgs.pop(slot, None)
+
+    def usage(self):
+        a = ctypes.c_int()
+        u = ctypes.c_size_t()
+        self._check(self.lib.shhd_batch_usage(self.h, ctypes.byref(a), ctypes.byref(u)))
+        return a.value, u.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.shhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
