@@ -107,6 +107,47 @@ def time_gpu(cfg, steps, warmup):
     return t1 - t0, kt, kb, info
 
 
+def ensemble_bench(ninst):
+    """Batched ensemble (configs[3] shape): ninst quiescent phi4 realizations
+    with sizes drawn from {64,128,256,512}^2, 100 SRK3 steps each (the stated
+    10^4 scaled down), dx = 1.5 for stability; all advanced together (one
+    launch per kernel, stage and launch shape, hipGraph replay) against the
+    same runs done one after another with single handles."""
+    import serve
+    jobs = serve.ensemble(ninst, nsteps=100)
+    srv = serve.JobServer(max_chunk=100, keep_results=False)
+    try:
+        rep = srv.run(jobs)
+    finally:
+        srv.close()
+    seq_s, _ = serve.run_sequential(jobs)
+    pts = rep.point_steps
+    return {"instances": ninst, "steps": 100, "Mpoint_steps_s": pts / rep.makespan / 1e6,
+            "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.makespan,
+            "batched_s": rep.makespan, "sequential_s": seq_s}
+
+
+def served_bench(njobs):
+    """Served stream (configs[4] shape, scaled down): njobs with Poisson
+    arrivals at 20 jobs/s, grids {256,512,1024,2048}^2 (weights .4/.3/.2/.1),
+    N in {0,1,2,4}, FP32 or FP64, steps log-uniform in [10, 200] (the stated
+    [1000, 20000] / 100); continuous batching with admission by footprint."""
+    import serve
+    jobs = serve.served_stream(njobs, rate=20.0, steps_range=(10, 200))
+    srv = serve.JobServer(max_chunk=32, keep_results=False)
+    try:
+        rep = srv.run(jobs)
+    finally:
+        srv.close()
+    p50, p99 = rep.latency_percentiles()
+    seq_s, _ = serve.run_sequential(jobs)
+    pts = rep.point_steps
+    return {"jobs": njobs, "arrival_rate_per_s": 20.0, "p50_s": p50, "p99_s": p99,
+            "Mpoint_steps_s": pts / rep.makespan / 1e6, "busy_Mpoint_steps_s": pts / rep.busy / 1e6,
+            "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.busy,
+            "makespan_s": rep.makespan, "chunks": rep.chunks, "peak_resident": rep.peak_resident}
+
+
 def cpu_baseline(cfg_gpu, budget_s):
     """FP64 CPU oracle (a restatement of the reference algorithm, own FFT,
     OpenMP over rows/columns) on the same workload: 1 untimed step, then whole
@@ -148,6 +189,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form side measurement")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--ensemble", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -213,6 +257,11 @@ def main():
         el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup)
         v32 = pts * args.steps / el32 / 1e6
         result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value}
+    if not args.no_serve:
+        result["ensemble"] = ensemble_bench(args.ensemble)
+        result["served"] = served_bench(args.serve_jobs)
+        result["p50_latency_s"] = result["served"]["p50_s"]
+        result["p99_latency_s"] = result["served"]["p99_s"]
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(result), flush=True)

@@ -1,0 +1,219 @@
+"""Request loop over the batched engine: continuous batching of simulation jobs.
+
+Jobs (model, grid, steps, seed, precision) arrive over time; the loop keeps
+every admitted job's state resident in HBM, admits queued jobs in arrival
+order whenever the device budget and slots allow (admission by footprint,
+`shhd_batch_footprint`), advances all resident jobs together in chunks of
+steps (one launch per kernel, stage and launch shape, replayed from captured
+hipGraphs), and retires each job as soon as its own step count is reached, so
+jobs that arrive while others run join at the next chunk boundary.
+
+Each job's result is bit-identical to running it alone (every instance uses
+its own Philox counters and the same kernels whatever else is resident).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+import shhd
+
+
+@dataclass
+class Job:
+    job_id: int
+    arrival: float                 # seconds after the start of the stream
+    cfg: shhd.SolverConfig
+    make_state: Callable[[], np.ndarray]   # initial state, built when the job is admitted
+    nsteps: int
+    # filled by the server
+    slot: int = -1
+    remaining: int = 0
+    admitted: float = -1.0
+    finished: float = -1.0
+    result: Optional[np.ndarray] = None
+
+    @property
+    def points(self) -> int:
+        return self.cfg.nx_active * self.cfg.ny_active
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class ServeReport:
+    jobs: List[Job]
+    makespan: float
+    chunks: int
+    peak_resident: int
+    busy: float = 0.0
+
+    @property
+    def point_steps(self) -> float:
+        return float(sum(j.points * j.nsteps for j in self.jobs))
+
+    def throughput(self) -> float:
+        """Mpoint*steps/s over the whole stream (first arrival to last finish)."""
+        return self.point_steps / self.makespan / 1e6
+
+    def latency_percentiles(self):
+        lat = np.array([j.latency for j in self.jobs])
+        return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+class JobServer:
+    def __init__(self, budget_bytes: int = 0, max_chunk: int = 48, use_graphs: bool = True,
+                 keep_results: bool = True, clock: Callable[[], float] = time.perf_counter):
+        self.batch = shhd.Batch(budget_bytes=budget_bytes, use_graphs=use_graphs)
+        self.budget = budget_bytes
+        self.max_chunk = max_chunk
+        self.keep_results = keep_results
+        self.clock = clock
+
+    def close(self):
+        self.batch.close()
+
+    def run(self, jobs: List[Job]) -> ServeReport:
+        pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
+        resident: List[Job] = []
+        done: List[Job] = []
+        t0 = self.clock()
+        chunks = 0
+        peak = 0
+        busy = 0.0
+        qi = 0
+        while len(done) < len(jobs):
+            now = self.clock() - t0
+            # admit arrived jobs in order while they fit
+            while qi < len(pending) and pending[qi].arrival <= now:
+                j = pending[qi]
+                try:
+                    j.slot = self.batch.admit(j.cfg, j.make_state())
+                except shhd.ShhdError as e:
+                    if e.status == shhd.ERR_FULL and resident:
+                        break          # wait for memory / slots to free up
+                    raise
+                j.remaining = j.nsteps
+                j.admitted = self.clock() - t0
+                resident.append(j)
+                qi += 1
+            peak = max(peak, len(resident))
+            if not resident:
+                # idle until the next arrival
+                wait = pending[qi].arrival - (self.clock() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                continue
+            chunk = min(self.max_chunk, min(j.remaining for j in resident))
+            tb = self.clock()
+            self.batch.step(chunk)
+            busy += self.clock() - tb
+            chunks += 1
+            still = []
+            for j in resident:
+                j.remaining -= chunk
+                if j.remaining == 0:
+                    if self.keep_results:
+                        j.result = self.batch.get_state(j.slot)
+                    self.batch.retire(j.slot)
+                    j.finished = self.clock() - t0
+                    done.append(j)
+                else:
+                    still.append(j)
+            resident = still
+        makespan = max(j.finished for j in jobs) - min(j.arrival for j in jobs)
+        return ServeReport(jobs=jobs, makespan=makespan, chunks=chunks, peak_resident=peak, busy=busy)
+
+
+def run_sequential(jobs: List[Job], clock: Callable[[], float] = time.perf_counter):
+    """The same jobs as plain single-simulation runs, one after another."""
+    t0 = clock()
+    out = []
+    for j in sorted(jobs, key=lambda j: (j.arrival, j.job_id)):
+        with shhd.GpuFluidSolver(j.cfg) as g:
+            g.set_state(j.make_state())
+            g.step(j.nsteps)
+            out.append(g.get_state())
+    return clock() - t0, out
+
+
+# ---------------------------------------------------------------------------
+# synthetic job streams
+# ---------------------------------------------------------------------------
+def random_state(cfg: shhd.SolverConfig, rng: np.random.Generator, psi_sigma: float = 0.01) -> np.ndarray:
+    """rho = 1, psi ~ N(0, psi_sigma^2) per point, small momentum noise;
+    masked and Hermitian like every reference initial condition."""
+    mask = shhd.active_mask(cfg)
+    m2 = cfg.nx * cfg.ny
+    nact = int(mask.sum())
+    st = np.zeros(cfg.state_shape(), np.complex128)
+    for f in range(cfg.num_fields):
+        amp = psi_sigma if 1 <= f <= cfg.num_order_parameters else (1e-3 if f > 0 else 0.0)
+        if amp == 0.0:
+            continue
+        z = (rng.standard_normal(mask.shape) + 1j * rng.standard_normal(mask.shape)) * mask
+        st[f] = z * (amp * m2 / math.sqrt(2.0 * nact))
+    st[:, 0, 0] = 0
+    st[0, 0, 0] = m2
+    return shhd.hermitian_project(st, cfg)
+
+
+def model_config(na: int, n: int, precision: int, seed: int, dt: float = 0.005) -> shhd.SolverConfig:
+    """Compressible NS (n = 0) or Model-H with n phi4 fields (a=-1, kappa=1, b=1),
+    dx = 1, 3/2 padding, SRK3, noise kBT = 1e-3."""
+    return shhd.SolverConfig(nx_active=na, ny_active=na, dealias="3/2", lx=float(na), ly=float(na), dt=dt,
+                             num_order_parameters=n, pressure_coefficient=1.0, eta=0.5, zeta=0.5,
+                             mobility=[1.0] * n, free_energy_kind=shhd.FREE_ENERGY_PHI4 if n else shhd.FREE_ENERGY_NULL,
+                             mu_k0=[-1.0] * n, mu_k2=[1.0] * n, phi4_u=[1.0] * n, momentum_advection=True,
+                             order_parameter_advection=n > 0, noise_momentum=True, noise_order_parameter=n > 0,
+                             kBT=1e-3, seed=seed, precision=precision)
+
+
+def quiescent_config(na: int, seed: int, precision: int = 64) -> shhd.SolverConfig:
+    """Quiescent conserved phi4 field (a=-0.5, kappa=1, b=1, M=1, kBT=0.05),
+    SRK3, dt = 0.02 with dx = 1.5 (dx = 1 is beyond the stability bound)."""
+    return shhd.SolverConfig(nx_active=na, ny_active=na, dealias="3/2", lx=1.5 * na, ly=1.5 * na, dt=0.02,
+                             mode=shhd.MODE_QUIESCENT, num_order_parameters=1, mobility=[1.0],
+                             free_energy_kind=shhd.FREE_ENERGY_PHI4, mu_k0=[-0.5], mu_k2=[1.0], phi4_u=[1.0],
+                             noise_order_parameter=True, kBT=0.05, seed=seed, precision=precision)
+
+
+def served_stream(njobs: int, rate: float, steps_range=(10, 200), seed: int = 5,
+                  sizes=(256, 512, 1024, 2048), weights=(0.4, 0.3, 0.2, 0.1), fields=(0, 1, 2, 4),
+                  precisions=(32, 64)) -> List[Job]:
+    """Poisson arrivals at `rate` jobs/s; grid, N, precision drawn per job;
+    steps log-uniform in steps_range."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        na = int(rng.choice(sizes, p=np.array(weights) / np.sum(weights)))
+        n = int(rng.choice(fields))
+        prec = int(rng.choice(precisions))
+        lo, hi = steps_range
+        nsteps = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        cfg = model_config(na, n, prec, seed=1000 + i)
+        sseed = int(rng.integers(1 << 31))
+        jobs.append(Job(job_id=i, arrival=t, cfg=cfg, nsteps=nsteps,
+                        make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed))))
+    return jobs
+
+
+def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512), seed: int = 7) -> List[Job]:
+    """Independent quiescent realizations of uneven sizes, all present at t = 0."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(ninst):
+        na = int(rng.choice(sizes))
+        cfg = quiescent_config(na, seed=seed0 + i)
+        sseed = int(rng.integers(1 << 31))
+        jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps,
+                        make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed), 0.1)))
+    return jobs

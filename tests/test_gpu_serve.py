@@ -1,0 +1,43 @@
+"""The request loop: continuous batching with admission by footprint; every
+job's result equals the same job run alone (bitwise), and latencies are
+reported."""
+import numpy as np
+import pytest
+
+import shhd
+import serve
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_stream_results_equal_alone():
+    print("[case] served stream, 14 jobs, budget-limited", flush=True)
+    jobs = serve.served_stream(14, rate=200.0, steps_range=(3, 40), sizes=(32, 64, 128), seed=3)
+    budget = 3 * max(shhd.Batch.footprint(j.cfg) for j in jobs)   # forces queueing
+    srv = serve.JobServer(budget_bytes=budget, max_chunk=8)
+    try:
+        rep = srv.run(jobs)
+    finally:
+        srv.close()
+    p50, p99 = rep.latency_percentiles()
+    print(f"    chunks {rep.chunks} peak resident {rep.peak_resident} p50 {p50*1e3:.1f} ms p99 {p99*1e3:.1f} ms",
+          flush=True)
+    assert rep.peak_resident >= 2
+    assert all(j.finished >= j.arrival for j in jobs)
+    _, alone = serve.run_sequential(jobs)
+    for j, a in zip(sorted(jobs, key=lambda j: (j.arrival, j.job_id)), alone):
+        assert np.array_equal(j.result, a), f"job {j.job_id}: mixed result differs from alone"
+
+
+def test_ensemble_uneven_sizes():
+    print("[case] ensemble of 12 uneven quiescent instances", flush=True)
+    jobs = serve.ensemble(12, nsteps=9, sizes=(32, 64, 128, 256))
+    srv = serve.JobServer(max_chunk=9)
+    try:
+        rep = srv.run(jobs)
+    finally:
+        srv.close()
+    assert rep.chunks == 1
+    _, alone = serve.run_sequential(jobs)
+    for j, a in zip(jobs, alone):
+        assert np.array_equal(j.result, a)
