@@ -146,22 +146,32 @@ def run_sequential(jobs: List[Job], clock: Callable[[], float] = time.perf_count
 # ---------------------------------------------------------------------------
 # synthetic job streams
 # ---------------------------------------------------------------------------
-def random_state(cfg: shhd.SolverConfig, rng: np.random.Generator, psi_sigma: float = 0.01) -> np.ndarray:
-    """rho = 1, psi ~ N(0, psi_sigma^2) per point, small momentum noise;
-    masked and Hermitian like every reference initial condition."""
-    mask = shhd.active_mask(cfg)
+def random_state(cfg: shhd.SolverConfig, rng: np.random.Generator, psi_sigma: float = 0.01,
+                 kmax: int = 32) -> np.ndarray:
+    """rho = 1, psi ~ N(0, psi_sigma^2) and a small momentum, both carried by
+    the modes with |kx|, |ky| < kmax (a long-wavelength random field, cheap to
+    build on the host for any grid); Hermitian on the kx = 0 column."""
     m2 = cfg.nx * cfg.ny
-    nact = int(mask.sum())
     st = np.zeros(cfg.state_shape(), np.complex128)
+    ny = cfg.ny
+    kx_n = min(cfg.nx_active // 2, kmax)
+    ky_n = min(cfg.ny_active // 2, kmax)
+    nmodes = kx_n * (2 * ky_n - 1)
     for f in range(cfg.num_fields):
         amp = psi_sigma if 1 <= f <= cfg.num_order_parameters else (1e-3 if f > 0 else 0.0)
         if amp == 0.0:
             continue
-        z = (rng.standard_normal(mask.shape) + 1j * rng.standard_normal(mask.shape)) * mask
-        st[f] = z * (amp * m2 / math.sqrt(2.0 * nact))
+        scale = amp * m2 / math.sqrt(2.0 * nmodes)
+        z = rng.standard_normal((2 * ky_n - 1, kx_n, 2))
+        v = (z[..., 0] + 1j * z[..., 1]) * scale
+        st[f, :ky_n, :kx_n] = v[:ky_n]
+        st[f, ny - ky_n + 1:, :kx_n] = v[ky_n:]
+        col = st[f][:, 0]
+        col[ny - ky_n + 1:] = np.conj(col[1:ky_n][::-1])
+        col[0] = col[0].real
     st[:, 0, 0] = 0
     st[0, 0, 0] = m2
-    return shhd.hermitian_project(st, cfg)
+    return st
 
 
 def model_config(na: int, n: int, precision: int, seed: int, dt: float = 0.005) -> shhd.SolverConfig:
@@ -185,11 +195,13 @@ def quiescent_config(na: int, seed: int, precision: int = 64) -> shhd.SolverConf
 
 
 def served_stream(njobs: int, rate: float, steps_range=(10, 200), seed: int = 5,
-                  sizes=(256, 512, 1024, 2048), weights=(0.4, 0.3, 0.2, 0.1), fields=(0, 1, 2, 4),
+                  sizes=(256, 512, 1024, 2048), weights=None, fields=(0, 1, 2, 4),
                   precisions=(32, 64)) -> List[Job]:
     """Poisson arrivals at `rate` jobs/s; grid, N, precision drawn per job;
     steps log-uniform in steps_range."""
     rng = np.random.default_rng(seed)
+    if weights is None:
+        weights = (0.4, 0.3, 0.2, 0.1) if len(sizes) == 4 else (1.0,) * len(sizes)
     t = 0.0
     jobs = []
     for i in range(njobs):
