@@ -145,10 +145,18 @@ shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n);
  * src/fluid_solver.h:97-99).  Returns after the device work has finished. */
 shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0);
 
-/* FluidSolver::set_flux_request / flux_buffer (src/fluid_solver.h:101-107).
- * Flux recording has no device path yet: a request with any flag set returns
- * SHHD_ERR_UNSUPPORTED; an all-zero request is accepted. */
+/* FluidSolver::set_flux_request (src/fluid_solver.h:101-103, FluxRequest of
+ * src/buffer_flux.h).  While any flag is set, every step records the
+ * step-averaged spectral fluxes of the requested conservation laws
+ * (FluxBuffer::accumulate_stage_flux with stage weights 1/6, 1/6, 2/3 for SRK3
+ * or 1 for Euler, draw A of the noise flux scaled by 1/sqrt(dt), and the
+ * incompressible projection pressure on Pi_xx and Pi_yy). */
 shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parameter, int momentum);
+/* FluidSolver::flux_buffer (src/fluid_solver.h:105-107): the flux of the last
+ * step as 2 + 2N + 3 slots [rho_x, rho_y, (psi_x, psi_y) per order parameter,
+ * Pi_xx, Pi_xy, Pi_yy] (src/buffer_flux.cc:10-48), each complex[(Mx/2+1)*My]
+ * in the state's layout, n = 2 * (2N+5) * (Mx/2+1) * My doubles.  Slots that
+ * are not requested read 0. */
 shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n);
 
 /* Destroy the handle and free its device memory. */
@@ -197,6 +205,9 @@ shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const dou
 /* advance every admitted instance by nsteps */
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps);
 shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n);
+/* flux recording of one instance, as shhd_gpu_set_flux_request / shhd_gpu_get_flux */
+shhd_status shhd_batch_set_flux_request(shhd_batch* b, int slot, int density, int order_parameter, int momentum);
+shhd_status shhd_batch_get_flux(shhd_batch* b, int slot, double* out, size_t n);
 /* free an instance's device memory and slot */
 shhd_status shhd_batch_retire(shhd_batch* b, int slot);
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes);

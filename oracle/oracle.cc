@@ -257,6 +257,18 @@ struct Oracle {
     double rho0 = 0.0;
 
     std::mt19937_64 rng;
+
+    // FluxBuffer (src/buffer_flux.{h,cc}): slots [rho x, rho y, (psi x, psi y) per
+    // order parameter, Pi xx, Pi xy, Pi yy], complex[spec] each
+    bool req_density = false, req_op = false, req_mom = false;
+    std::vector<cplx> flux_total, flux_det, flux_sto;
+    std::vector<cplx>* rec = nullptr;   // buffer the RHS terms record into
+    int nslots() const { return 2 + 2 * N + 3; }
+    bool flux_requested() const { return req_density || req_op || req_mom; }
+    cplx* fslot(int k) { return rec->data() + static_cast<size_t>(k) * spec; }
+    cplx* f_rho(int c) { return fslot(c); }
+    cplx* f_op(int a, int c) { return fslot(2 + 2 * a + c); }
+    cplx* f_pi(int c) { return fslot(2 + 2 * N + c); }   // 0 xx, 1 xy, 2 yy
     std::normal_distribution<double> normal{0.0, 1.0};
 
     double kx_of(int gx) const { return 2.0 * kPi * static_cast<double>(gx) / cfg.lx; }
@@ -372,6 +384,8 @@ struct Oracle {
         jinc_y.assign(spec, cplx(0, 0));
         spectral_pack.assign(static_cast<size_t>(ntransform) * spec, cplx(0, 0));
         physical_pack.assign(static_cast<size_t>(nphysical) * phys, 0.0);
+        flux_total.assign(static_cast<size_t>(nslots()) * spec, cplx(0, 0));
+        flux_det = flux_sto = flux_total;
         const int cap = std::max(N, 9);   // src/fcalculator.cc:11-19
         tmp_phys.assign(static_cast<size_t>(cap) * phys, 0.0);
         tmp_spec.assign(static_cast<size_t>(cap) * spec, cplx(0, 0));
@@ -454,6 +468,12 @@ struct Oracle {
         const cplx* jx = field(s, jx_field());
         const cplx* jy = field(s, jy_field());
         for (const Mode& m : active) out[m.index] = -I * (m.kx * jx[m.index] + m.ky * jy[m.index]);
+        if (rec && req_density) {
+            for (const Mode& m : active) {
+                f_rho(0)[m.index] += jx[m.index];
+                f_rho(1)[m.index] += jy[m.index];
+            }
+        }
     }
 
     void psi_det(int a, const cplx* s, cplx* out, double t) {   // fcalculator.cc:86-108
@@ -477,6 +497,10 @@ struct Oracle {
             const cplx fx = -cplx(0.0, mob * m.kx) * mu;
             const cplx fy = -cplx(0.0, mob * m.ky) * mu;
             out[m.index] += -I * (m.kx * fx + m.ky * fy);
+            if (rec && req_op) {
+                f_op(a, 0)[m.index] += fx;
+                f_op(a, 1)[m.index] += fy;
+            }
         }
     }
 
@@ -520,12 +544,22 @@ struct Oracle {
                 const cplx fx = -cplx(0.0, mob * m.kx) * mu[m.index];
                 const cplx fy = -cplx(0.0, mob * m.ky) * mu[m.index];
                 out[m.index] += -I * (m.kx * fx + m.ky * fy);
+                if (rec && req_op) {
+                    f_op(a, 0)[m.index] += fx;
+                    f_op(a, 1)[m.index] += fy;
+                }
             }
         }
         if (adv) {
             const cplx* fx = sf + static_cast<size_t>(f_x) * spec;
             const cplx* fy = sf + static_cast<size_t>(f_y) * spec;
-            for (const Mode& m : active) out[m.index] += -I * (m.kx * fx[m.index] + m.ky * fy[m.index]);
+            for (const Mode& m : active) {
+                out[m.index] += -I * (m.kx * fx[m.index] + m.ky * fy[m.index]);
+                if (rec && req_op) {
+                    f_op(a, 0)[m.index] += fx[m.index];
+                    f_op(a, 1)[m.index] += fy[m.index];
+                }
+            }
         }
     }
 
@@ -552,6 +586,10 @@ struct Oracle {
             const cplx p = pc * rho[m.index];
             ox[m.index] += -cplx(0.0, m.kx) * p;
             oy[m.index] += -cplx(0.0, m.ky) * p;
+            if (rec && req_mom) {
+                f_pi(0)[m.index] += p;
+                f_pi(2)[m.index] += p;
+            }
         }
     }
 
@@ -568,6 +606,11 @@ struct Oracle {
                 const cplx pyy = -cplx(0.0, eta) * (2.0 * m.ky * vy);
                 ox[m.index] += -I * (m.kx * pxx + m.ky * pxy);
                 oy[m.index] += -I * (m.kx * pxy + m.ky * pyy);
+                if (rec && req_mom) {
+                    f_pi(0)[m.index] += pxx;
+                    f_pi(1)[m.index] += pxy;
+                    f_pi(2)[m.index] += pyy;
+                }
             }
         }
     }
@@ -589,6 +632,11 @@ struct Oracle {
             const cplx pyy = -cplx(0.0, 1) * (2.0 * eta * m.ky * vy[i] - (eta - zeta) * kv);
             ox[i] += -I * (m.kx * pxx + m.ky * pxy);
             oy[i] += -I * (m.kx * pxy + m.ky * pyy);
+            if (rec && req_mom) {
+                f_pi(0)[i] += pxx;
+                f_pi(1)[i] += pxy;
+                f_pi(2)[i] += pyy;
+            }
         }
     }
 
@@ -615,6 +663,11 @@ struct Oracle {
             const cplx a = sf[i], b = sf[spec + i], c = sf[2 * spec + i];
             ox[i] += -I * (m.kx * a + m.ky * b);
             oy[i] += -I * (m.kx * b + m.ky * c);
+            if (rec && req_mom) {
+                f_pi(0)[i] += a;
+                f_pi(1)[i] += b;
+                f_pi(2)[i] += c;
+            }
         }
     }
 
@@ -709,6 +762,12 @@ struct Oracle {
             const size_t ic = static_cast<size_t>(cgy) * hx;
             out[i] += value;
             out[ic] += std::conj(value);
+            if (rec && req_op) {
+                f_op(a, 0)[i] += fx;
+                f_op(a, 1)[i] += fy;
+                f_op(a, 0)[ic] += std::conj(fx);
+                f_op(a, 1)[ic] += std::conj(fy);
+            }
         }
         for (const Mode& m : active) {
             if (m.gx == 0 || m.k2 == 0.0) continue;
@@ -719,6 +778,10 @@ struct Oracle {
             const cplx fx = cplx(0.0, m.kx / m.k2) * sigma * xi;
             const cplx fy = cplx(0.0, m.ky / m.k2) * sigma * xi;
             out[m.index] += -I * (m.kx * fx + m.ky * fy);
+            if (rec && req_op) {
+                f_op(a, 0)[m.index] += fx;
+                f_op(a, 1)[m.index] += fy;
+            }
         }
     }
 
@@ -750,6 +813,10 @@ struct Oracle {
             const size_t ic = static_cast<size_t>(cgy) * hx;
             ox[i] += vx; oy[i] += vy;
             ox[ic] += std::conj(vx); oy[ic] += std::conj(vy);
+            if (rec && req_mom) {
+                f_pi(0)[i] += pxx; f_pi(1)[i] += pxy; f_pi(2)[i] += pyy;
+                f_pi(0)[ic] += std::conj(pxx); f_pi(1)[ic] += std::conj(pxy); f_pi(2)[ic] += std::conj(pyy);
+            }
         }
         for (const Mode& m : active) {
             if (m.gx == 0 || m.k2 == 0.0) continue;
@@ -759,6 +826,45 @@ struct Oracle {
             const cplx pxy = -ss * se * x3;
             ox[m.index] += -I * (m.kx * pxx + m.ky * pxy);
             oy[m.index] += -I * (m.kx * pxy + m.ky * pyy);
+            if (rec && req_mom) {
+                f_pi(0)[m.index] += pxx;
+                f_pi(1)[m.index] += pxy;
+                f_pi(2)[m.index] += pyy;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------------
+    // FluxBuffer accumulation (src/buffer_flux.cc:170-225)
+    // ---------------------------------------------------------------------
+    void flux_begin(std::vector<cplx>& b) {
+        if (flux_requested()) std::fill(b.begin(), b.end(), cplx(0, 0));
+    }
+    void flux_accumulate(double w, const std::vector<cplx>& src) {
+        if (!flux_requested() || w == 0.0) return;
+        auto acc = [&](int k) {
+            cplx* d = flux_total.data() + static_cast<size_t>(k) * spec;
+            const cplx* v = src.data() + static_cast<size_t>(k) * spec;
+            for (size_t i = 0; i < spec; ++i) d[i] += w * v[i];
+        };
+        if (req_density) { acc(0); acc(1); }
+        if (req_op) for (int a = 0; a < N; ++a) { acc(2 + 2 * a); acc(3 + 2 * a); }
+        if (req_mom) { acc(2 + 2 * N); acc(3 + 2 * N); acc(4 + 2 * N); }
+    }
+    void flux_projection_pressure() {
+        if (!req_mom) return;
+        cplx* pxx = flux_total.data() + static_cast<size_t>(2 + 2 * N) * spec;
+        cplx* pxy = pxx + spec;
+        cplx* pyy = pxy + spec;
+        for (const Mode& m : active) {
+            if (m.k2 == 0.0) continue;
+            const size_t i = m.index;
+            const cplx rx = -I * (m.kx * pxx[i] + m.ky * pxy[i]);
+            const cplx ry = -I * (m.kx * pxy[i] + m.ky * pyy[i]);
+            const cplx kr = m.kx * rx + m.ky * ry;
+            const cplx p = -I * kr / m.k2;
+            pxx[i] += p;
+            pyy[i] += p;
         }
     }
 
@@ -808,10 +914,13 @@ struct Oracle {
     // SRK3 stage (src/time_integrator_srk3_{compressible,incompressible,quiescent}.h)
     // ---------------------------------------------------------------------
     void stage(std::vector<cplx>& next, const std::vector<cplx>& cur, const std::vector<cplx>& base,
-               double t, double beta, double wb, double wc) {
+               double t, double beta, double wb, double wc, double flux_weight) {
         const bool compressible = dyn == Dyn::Compressible;
         if (!compressible && &next != &cur) next = cur;   // copy_state(next, current)
+        flux_begin(flux_det);
+        rec = flux_requested() ? &flux_det : nullptr;
         eval_det(cur, t, compressible);
+        rec = nullptr;
         const cplx* cd = cur.data();
         const cplx* bd = base.data();
         const cplx* dd = det.data();
@@ -859,6 +968,7 @@ struct Oracle {
             }
         }
         enforce_real_symmetry(next);
+        flux_accumulate(flux_weight, flux_det);
     }
 
     void step_srk3(double t) {
@@ -868,42 +978,62 @@ struct Oracle {
         static constexpr double B2 = (-4.0 * S2 + 3.0 * S3) / 5.0;
         static constexpr double B3 = (S2 - 2.0 * S3) / 10.0;
         u_old = state;
+        flux_begin(flux_total);
+        flux_begin(flux_sto);
         std::fill(sto_a.begin(), sto_a.end(), cplx(0, 0));
         std::fill(sto_b.begin(), sto_b.end(), cplx(0, 0));
         // draw order: psi A/B per order parameter, then j A, j B
         // (time_integrator_srk3_compressible.h:116-126)
+        std::vector<cplx>* sto_rec = flux_requested() ? &flux_sto : nullptr;
         if (has_psi_sto) {
             for (int a = 0; a < N; ++a) {
+                rec = sto_rec;
                 psi_sto(a, field(sto_a, psi_field(a)));
+                rec = nullptr;
                 psi_sto(a, field(sto_b, psi_field(a)));
             }
         }
         if (has_j_sto) {
+            rec = sto_rec;
             j_sto(field(sto_a, jx_field()), field(sto_a, jy_field()));
+            rec = nullptr;
             j_sto(field(sto_b, jx_field()), field(sto_b, jy_field()));
         }
-        stage(u1, u_old, u_old, t, B1, 0.0, 1.0);
-        stage(u2, u1, u_old, t + dt, B2, 3.0 / 4.0, 1.0 / 4.0);
-        stage(state, u2, u_old, t + 0.5 * dt, B3, 1.0 / 3.0, 2.0 / 3.0);
+        stage(u1, u_old, u_old, t, B1, 0.0, 1.0, 1.0 / 6.0);
+        stage(u2, u1, u_old, t + dt, B2, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 6.0);
+        stage(state, u2, u_old, t + 0.5 * dt, B3, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0);
+        flux_accumulate(1.0 / sqrt_dt, flux_sto);
+        if (dyn == Dyn::Incompressible) flux_projection_pressure();
     }
 
     // Euler (src/time_integrator_euler_*.h): one draw, interleaved with det
     // per order parameter exactly as the reference loops.
     void step_euler(double t) {
         u_old = state;
+        flux_begin(flux_total);
+        flux_begin(flux_det);
+        flux_begin(flux_sto);
+        std::vector<cplx>* det_rec = flux_requested() ? &flux_det : nullptr;
+        std::vector<cplx>* sto_rec = flux_requested() ? &flux_sto : nullptr;
         std::fill(det.begin(), det.end(), cplx(0, 0));
         std::fill(sto_a.begin(), sto_a.end(), cplx(0, 0));
         const cplx* c = u_old.data();
         const bool compressible = dyn == Dyn::Compressible;
+        rec = det_rec;
         if (compressible) rho_det(c, field(det, 0));
         for (int a = 0; a < N; ++a) {
+            rec = det_rec;
             psi_det(a, c, field(det, psi_field(a)), t);
+            rec = sto_rec;
             if (has_psi_sto) psi_sto(a, field(sto_a, psi_field(a)));
         }
         if (dyn != Dyn::Quiescent) {
+            rec = det_rec;
             j_det(c, field(det, jx_field()), field(det, jy_field()), t);
+            rec = sto_rec;
             if (has_j_sto) j_sto(field(sto_a, jx_field()), field(sto_a, jy_field()));
         }
+        rec = nullptr;
         cplx* nd = state.data();
         const cplx* dd = det.data();
         const cplx* sd = sto_a.data();
@@ -944,6 +1074,9 @@ struct Oracle {
             }
         }
         enforce_real_symmetry(state);
+        flux_accumulate(1.0, flux_det);
+        flux_accumulate(1.0 / sqrt_dt, flux_sto);
+        if (dyn == Dyn::Incompressible) flux_projection_pressure();
     }
 };
 
@@ -1038,6 +1171,21 @@ int oracle_rhs_sto(oracle_solver* s, double* out) {
     Oracle& o = *s->impl;
     o.draw_noise(o.sto_a);
     std::memcpy(out, o.sto_a.data(), sizeof(cplx) * o.sto_a.size());
+    return SHHD_OK;
+}
+
+int oracle_set_flux_request(oracle_solver* s, int density, int order_parameter, int momentum) {
+    Oracle& o = *s->impl;
+    o.req_density = density != 0;
+    o.req_op = order_parameter != 0 && o.N > 0;
+    o.req_mom = momentum != 0;
+    return SHHD_OK;
+}
+
+int oracle_get_flux(const oracle_solver* s, double* out, size_t n) {
+    const Oracle& o = *s->impl;
+    if (n != 2 * o.flux_total.size()) return SHHD_ERR_INVALID;
+    std::memcpy(out, o.flux_total.data(), sizeof(double) * n);
     return SHHD_OK;
 }
 

@@ -37,6 +37,11 @@ int  oracle_rhs_det(oracle_solver* s, const double* state, double* out, double t
  * solver's mt19937_64 stream. */
 int  oracle_rhs_sto(oracle_solver* s, double* out);
 
+/* FluxBuffer (src/buffer_flux.{h,cc}): request and read the step-averaged
+ * spectral fluxes, slots [rho x, rho y, (psi x, psi y)*N, Pi xx, Pi xy, Pi yy]. */
+int  oracle_set_flux_request(oracle_solver* s, int density, int order_parameter, int momentum);
+int  oracle_get_flux(const oracle_solver* s, double* out, size_t n);
+
 /* Transforms used by the oracle (pinned against the DFT definition). */
 int  oracle_fft_c2c(int n, const double* in, double* out, int sign);
 int  oracle_fft2_r2c(int mx, int my, const double* phys, double* spec);

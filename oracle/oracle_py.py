@@ -42,6 +42,8 @@ def load():
     lib.oracle_step.argtypes = [P, ctypes.c_int, ctypes.c_double]
     lib.oracle_rhs_det.argtypes = [P, D, D, ctypes.c_double]
     lib.oracle_rhs_sto.argtypes = [P, D]
+    lib.oracle_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.oracle_get_flux.argtypes = [P, D, ctypes.c_size_t]
     lib.oracle_fft_c2c.argtypes = [ctypes.c_int, D, D, ctypes.c_int]
     lib.oracle_fft2_r2c.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
     lib.oracle_fft2_c2r.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
@@ -103,6 +105,15 @@ class Oracle:
         s = np.ascontiguousarray(state, dtype=np.complex128)
         out = np.zeros_like(s)
         self._check(self.lib.oracle_rhs_det(self.h, _d(s.view(np.float64)), _d(out.view(np.float64)), float(t)))
+        return out
+
+    def set_flux_request(self, density=False, order_parameter=False, momentum=False):
+        self._check(self.lib.oracle_set_flux_request(self.h, int(density), int(order_parameter), int(momentum)))
+
+    def get_flux(self):
+        out = np.zeros(self.cfg.flux_shape(), dtype=np.complex128)
+        f = out.view(np.float64).reshape(-1)
+        self._check(self.lib.oracle_get_flux(self.h, _d(f), f.size))
         return out
 
     def rhs_sto(self):

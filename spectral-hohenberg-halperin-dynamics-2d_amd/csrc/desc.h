@@ -120,7 +120,16 @@ struct StageParams {
     double beta, wb, wc;
     int use_b;              // SRK3: A + beta B; Euler: A only
     uint32_t step_lo, step_hi;   // filled in-kernel from the instance's step counter
+    // flux recording (src/buffer_flux.cc:170-194, time_integrator_*.h): weight of
+    // this stage's deterministic flux; the first stage clears the step's flux
+    // and adds draw A of the noise flux scaled by 1/sqrt(dt); the last stage
+    // adds the incompressible projection pressure
+    double flux_w;
+    int flux_first, flux_last;
 };
+
+// Flux request bits (FluxRequest, src/buffer_flux.h)
+enum FluxBits : int { FLUX_DENSITY = 1, FLUX_ORDER_PARAMETER = 2, FLUX_MOMENTUM = 4 };
 
 // Everything one instance needs on the device, kept in a device-side table so
 // that one launch can serve many instances of different sizes (batched
@@ -135,6 +144,9 @@ struct InstDev {
     cx<T>* ubuf;              // K2 -> K3 planes
     int* flag;                // zero-density flag
     const uint64_t* step;     // Philox step counter of this instance
+    cx<T>* flux;              // step flux, compact slots [rho x, rho y, (psi x, psi y)*N,
+                              // Pi xx, Pi xy, Pi yy]; null unless requested
+    int flux_req;             // FluxBits
     int C;                    // kx columns per column-pass workgroup
     int R;                    // rows per row-pass workgroup
     int ntiles;               // column tiles = ceil(Nax/2 / C)

@@ -326,6 +326,39 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     const int tid = threadIdx.x;
     const double two_pi = 2.0 * 3.14159265358979323846;
 
+    // ---- flux slots this group records (FluxBuffer, src/buffer_flux.h) -------
+    // Every mode is read and written by the one lane that owns it in all the
+    // loops below (m = tid + NT * i), so the accumulation needs no atomics.
+    const size_t fe = static_cast<size_t>(hxa) * nay;
+    cx<T>* fx_slot = nullptr;   // psi group: psi flux x; momentum group: rho flux x
+    cx<T>* fy_slot = nullptr;
+    cx<T>* pi_slot = nullptr;   // Pi xx, then Pi xy and Pi yy at +fe, +2fe
+    if (I.flux) {
+        if (g.kind == GROUP_PSI) {
+            if (I.flux_req & FLUX_ORDER_PARAMETER) {
+                fx_slot = I.flux + static_cast<size_t>(2 + 2 * g.alpha) * fe;
+                fy_slot = fx_slot + fe;
+            }
+        } else {
+            if ((I.flux_req & FLUX_DENSITY) && d.mode == SHHD_MODE_COMPRESSIBLE) {
+                fx_slot = I.flux;
+                fy_slot = I.flux + fe;
+            }
+            if (I.flux_req & FLUX_MOMENTUM) pi_slot = I.flux + static_cast<size_t>(2 + 2 * d.N) * fe;
+        }
+        if (sp.flux_first) {   // FluxBuffer::begin_step
+            for (int m = tid; m < nmodes; m += NT) {
+                const int c = m / nay;
+                const int gx = gx0 + c;
+                if (gx >= hxa) continue;
+                const size_t at = static_cast<size_t>(gx) * nay + (m - c * nay);
+                if (fx_slot) { fx_slot[at] = czero<T>(); fy_slot[at] = czero<T>(); }
+                if (pi_slot) { pi_slot[at] = czero<T>(); pi_slot[at + fe] = czero<T>(); pi_slot[at + 2 * fe] = czero<T>(); }
+            }
+        }
+    }
+    const T fw = T(sp.flux_w);
+
     // ACC >= ceil(C * Nay / NT) accumulator slots per lane (host-checked)
     cx<T> acc0[ACC], acc1[ACC];
 #pragma unroll
@@ -357,24 +390,40 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             const cx<T> v = lds[lp(c * L + gy)];
             const T kx = T(two_pi * gx / d.lx);
             const T ky = T(two_pi * ky_i / d.ly);
+            const size_t at = static_cast<size_t>(gx) * nay + cidx;
             switch (kind) {
                 case OUT_MU: {   // src/fcalculator.cc:244-256
                     const T mob = T(d.mobility[g.alpha]);
                     const cx<T> fx = mul_mi((mob * kx) * v);
                     const cx<T> fy = mul_mi((mob * ky) * v);
                     acc0[i] = acc0[i] + mul_mi(kx * fx + ky * fy);
+                    if (fx_slot) {
+                        fx_slot[at] = fx_slot[at] + fw * fx;
+                        fy_slot[at] = fy_slot[at] + fw * fy;
+                    }
                     break;
                 }
-                case OUT_PSIVX: acc0[i] = acc0[i] + mul_mi(kx * v); break;
-                case OUT_PSIVY: acc0[i] = acc0[i] + mul_mi(ky * v); break;
-                case OUT_GRADF: acc0[i] = acc0[i] + v; break;
-                case OUT_VX: {   // src/fcalculator.cc:378-392 with vy = 0
+                case OUT_PSIVX:   // src/fcalculator.cc:258-272
+                    acc0[i] = acc0[i] + mul_mi(kx * v);
+                    if (fx_slot) fx_slot[at] = fx_slot[at] + fw * v;
+                    break;
+                case OUT_PSIVY:
+                    acc0[i] = acc0[i] + mul_mi(ky * v);
+                    if (fy_slot) fy_slot[at] = fy_slot[at] + fw * v;
+                    break;
+                case OUT_GRADF: acc0[i] = acc0[i] + v; break;   // forces record no flux
+                case OUT_VX: {   // src/fcalculator.cc:378-397 with vy = 0
                     const T eta = T(d.eta), zeta = T(d.zeta);
                     const cx<T> pxx = mul_mi((T(2) * eta * kx - (eta - zeta) * kx) * v);
                     const cx<T> pxy = mul_mi((eta * ky) * v);
                     const cx<T> pyy = mul_mi((-(eta - zeta) * kx) * v);
                     acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
                     acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
+                    if (pi_slot) {
+                        pi_slot[at] = pi_slot[at] + fw * pxx;
+                        pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
+                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
+                    }
                     break;
                 }
                 case OUT_VY: {   // same with vx = 0
@@ -384,14 +433,26 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     const cx<T> pyy = mul_mi((T(2) * eta * ky - (eta - zeta) * ky) * v);
                     acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
                     acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
+                    if (pi_slot) {
+                        pi_slot[at] = pi_slot[at] + fw * pxx;
+                        pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
+                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
+                    }
                     break;
                 }
-                case OUT_PXX: acc0[i] = acc0[i] + mul_mi(kx * v); break;
+                case OUT_PXX:   // src/fcalculator.cc:472-491
+                    acc0[i] = acc0[i] + mul_mi(kx * v);
+                    if (pi_slot) pi_slot[at] = pi_slot[at] + fw * v;
+                    break;
                 case OUT_PXY:
                     acc0[i] = acc0[i] + mul_mi(ky * v);
                     acc1[i] = acc1[i] + mul_mi(kx * v);
+                    if (pi_slot) pi_slot[at + fe] = pi_slot[at + fe] + fw * v;
                     break;
-                default: acc1[i] = acc1[i] + mul_mi(ky * v); break;   // OUT_PYY
+                default:   // OUT_PYY
+                    acc1[i] = acc1[i] + mul_mi(ky * v);
+                    if (pi_slot) pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * v;
+                    break;
             }
         }
         __syncthreads();
@@ -458,6 +519,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     const cx<T> fx = mul_mi((mob * kx) * mu);
                     const cx<T> fy = mul_mi((mob * ky) * mu);
                     det = det + mul_mi(kx * fx + ky * fy);
+                    if (fx_slot) {
+                        fx_slot[at] = fx_slot[at] + fw * fx;
+                        fy_slot[at] = fy_slot[at] + fw * fy;
+                    }
                 }
                 for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
                     if (d.igrad_component[q] != a) continue;
@@ -474,12 +539,19 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
                 const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
                 const cx<T> xa = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 0u, sp);
-                cx<T> na = sigma * xa;
+                const cx<T> sa = sigma * xa;
+                cx<T> na = sa;
                 if (sp.use_b) {
                     const cx<T> xb = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 1u, sp);
                     na = na + beta * (sigma * xb);
                 }
                 sto = conj_draw ? conj(na) : na;
+                if (fx_slot && sp.flux_first) {   // draw A: (i k / k^2) sigma xi, times 1/sqrt(dt)
+                    const cx<T> fa = conj_draw ? conj(sa) : sa;
+                    const T isd = T(1.0 / d.sqrt_dt);
+                    fx_slot[at] = fx_slot[at] + isd * mul_i((kx / k2) * fa);
+                    fy_slot[at] = fy_slot[at] + isd * mul_i((ky / k2) * fa);
+                }
             }
             if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
             else P.next[f][at] = wb * P.base[f][at] + wc * (cur + dt * det + sdt * sto);
@@ -492,6 +564,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     const cx<T> p = T(d.pressure_coefficient) * P.cur[0][at];
                     dx = dx + mul_mi(kx * p);
                     dy = dy + mul_mi(ky * p);
+                    if (pi_slot) {
+                        pi_slot[at] = pi_slot[at] + fw * p;
+                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * p;
+                    }
                 }
                 if (d.incompressible_viscous) {   // src/fcalculator.cc:302-341
                     const T eta = T(d.eta);
@@ -503,6 +579,11 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     const cx<T> pyy = mul_mi(eta * ((T(2) * ky) * vy));
                     dx = dx + mul_mi(kx * pxx + ky * pxy);
                     dy = dy + mul_mi(kx * pxy + ky * pyy);
+                    if (pi_slot) {
+                        pi_slot[at] = pi_slot[at] + fw * pxx;
+                        pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
+                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
+                    }
                 }
                 for (int q = 0; q < d.nsources; ++q) {
                     if (d.sources[q].gx != gx || d.sources[q].cidx != cidx) continue;
@@ -527,12 +608,33 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     cx<T> vx = mul_mi(kxd * pxx + kyd * pxy);
                     cx<T> vy = mul_mi(kxd * pxy + kyd * pyy);
                     if (conj_draw) { vx = conj(vx); vy = conj(vy); }
+                    if (ab == 0 && pi_slot && sp.flux_first) {   // draw A stress, times 1/sqrt(dt)
+                        const T isd = T(1.0 / d.sqrt_dt);
+                        pi_slot[at] = pi_slot[at] + isd * (conj_draw ? conj(pxx) : pxx);
+                        pi_slot[at + fe] = pi_slot[at + fe] + isd * (conj_draw ? conj(pxy) : pxy);
+                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
+                    }
                     if (ab == 0) { sx = vx; sy = vy; }
                     else { sx = sx + beta * vx; sy = sy + beta * vy; }
                 }
             }
+            if (pi_slot && sp.flux_last && !compressible && k2 != T(0)) {
+                // FluxBuffer::add_incompressible_projection_pressure_flux
+                // (src/buffer_flux.cc:196-225) on the step's accumulated stress
+                const cx<T> txx = pi_slot[at], txy = pi_slot[at + fe], tyy = pi_slot[at + 2 * fe];
+                const cx<T> rx = mul_mi(kx * txx + ky * txy);
+                const cx<T> ry = mul_mi(kx * txy + ky * tyy);
+                const cx<T> kr = kx * rx + ky * ry;
+                const cx<T> pk = mul_mi(mk(kr.re / k2, kr.im / k2));
+                pi_slot[at] = txx + pk;
+                pi_slot[at + 2 * fe] = tyy + pk;
+            }
             if (compressible) {
                 const cx<T> drho = mul_mi(kx * jx + ky * jy);   // src/fcalculator.cc:60-84
+                if (fx_slot) {
+                    fx_slot[at] = fx_slot[at] + fw * jx;
+                    fy_slot[at] = fy_slot[at] + fw * jy;
+                }
                 if (is_k0) {
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[0] + at, 0, dt * drho, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);

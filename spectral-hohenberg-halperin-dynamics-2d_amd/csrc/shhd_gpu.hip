@@ -267,8 +267,14 @@ struct Instance {
     int cur = 0;
     uint64_t step_index = 0;
     size_t bytes = 0;
+    // flux recording (FluxBuffer, src/buffer_flux.h): slots of field_elems each
+    void* flux = nullptr;
+    int flux_req = 0;           // FluxBits
+    bool flux_clear = false;    // zero every slot before the next step
+    int flux_slots() const { return 2 + 2 * desc.N + 3; }
 
     ~Instance() {
+        if (flux) (void)hipFree(flux);
         for (int b = 0; b < 3; ++b) {
             if (state[b]) (void)hipFree(state[b]);
             if (k0[b]) (void)hipFree(k0[b]);
@@ -681,6 +687,29 @@ shhd_status upload_state(Instance* h, const double* in, size_t n, hipStream_t st
     return SHHD_OK;
 }
 
+// Compact per-field columns -> reference layout (active modes only; the rest
+// of `out` is left as the caller cleared it).
+void compact_to_reference(const Instance& h, const cx<double>* compact, int nf, double* out) {
+    const shhd_config& c = h.cfg;
+    const size_t spec = ref_spec(c);
+    const int hx = c.nx / 2 + 1;
+    const int nay = c.ny_active;
+    for (int f = 0; f < nf; ++f) {
+        double* dst = out + 2 * spec * static_cast<size_t>(f);
+        for (int gy = 0; gy < c.ny; ++gy) {
+            for (int gx = 0; gx < c.nx_active / 2; ++gx) {
+                if (!ref_active(c, gx, gy)) continue;
+                const int ky = (gy <= c.ny / 2) ? gy : gy - c.ny;
+                const int cidx = ky >= 0 ? ky : ky + nay;
+                const cx<double> v = compact[static_cast<size_t>(f) * h.field_elems + static_cast<size_t>(gx) * nay + cidx];
+                const size_t i = static_cast<size_t>(gy) * hx + gx;
+                dst[2 * i] = v.re;
+                dst[2 * i + 1] = v.im;
+            }
+        }
+    }
+}
+
 shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t stream, std::string* err) {
     const shhd_config& c = h->cfg;
     const size_t spec = ref_spec(c);
@@ -705,22 +734,7 @@ shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t
         }
     }
     std::memset(out, 0, sizeof(double) * n);
-    const int hx = c.nx / 2 + 1;
-    const int nay = c.ny_active;
-    for (int f = 0; f < nf; ++f) {
-        double* dst = out + 2 * spec * static_cast<size_t>(f);
-        for (int gy = 0; gy < c.ny; ++gy) {
-            for (int gx = 0; gx < c.nx_active / 2; ++gx) {
-                if (!ref_active(c, gx, gy)) continue;
-                const int ky = (gy <= c.ny / 2) ? gy : gy - c.ny;
-                const int cidx = ky >= 0 ? ky : ky + nay;
-                const cx<double> v = compact[static_cast<size_t>(f) * h->field_elems + static_cast<size_t>(gx) * nay + cidx];
-                const size_t i = static_cast<size_t>(gy) * hx + gx;
-                dst[2 * i] = v.re;
-                dst[2 * i + 1] = v.im;
-            }
-        }
-    }
+    compact_to_reference(*h, compact.data(), nf, out);
     return SHHD_OK;
 }
 
@@ -865,6 +879,56 @@ struct Engine {
         return slots[static_cast<size_t>(slot)].get();
     }
 
+    // FluidSolver::set_flux_request (src/fluid_solver.h:101-103).  The buffer
+    // is allocated on the first request; a changed request clears every slot
+    // before the next step, as FluxBuffer::begin_step does.
+    shhd_status set_flux_request(int slot, int density, int order_parameter, int momentum) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        const int req = (density ? FLUX_DENSITY : 0) | (order_parameter ? FLUX_ORDER_PARAMETER : 0) |
+                        (momentum ? FLUX_MOMENTUM : 0);
+        if (req && !h->flux) {
+            const size_t fb = static_cast<size_t>(h->flux_slots()) * h->field_elems * h->elem;
+            if (budget && used + fb > budget)
+                return set_err(&err, SHHD_ERR_FULL, "device budget exhausted: flux buffer needs " + std::to_string(fb) + " bytes");
+            SHHD_HIP(&err, hipStreamSynchronize(stream));
+            SHHD_HIP(&err, hipMalloc(&h->flux, fb));
+            SHHD_HIP(&err, hipMemset(h->flux, 0, fb));
+            h->bytes += fb;
+            used += fb;
+        }
+        if (req != h->flux_req) {
+            if (req) h->flux_clear = true;
+            h->flux_req = req;
+            dirty = true;
+        }
+        return SHHD_OK;
+    }
+
+    // FluidSolver::flux_buffer (src/fluid_solver.h:105-107) in the reference
+    // layout: slots of complex[(Mx/2+1) * My], inactive modes 0.
+    shhd_status get_flux(int slot, double* out, size_t n) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        const shhd_config& c = h->cfg;
+        const size_t spec = ref_spec(c);
+        const int ns = h->flux_slots();
+        if (n != 2 * spec * static_cast<size_t>(ns)) return set_err(&err, SHHD_ERR_INVALID, "flux size mismatch");
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        std::memset(out, 0, sizeof(double) * n);
+        if (!h->flux) return SHHD_OK;
+        std::vector<cx<double>> compact(static_cast<size_t>(ns) * h->field_elems);
+        if (h->fp32) {
+            std::vector<cx<float>> cf(compact.size());
+            SHHD_HIP(&err, hipMemcpy(cf.data(), h->flux, cf.size() * sizeof(cx<float>), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < cf.size(); ++i) compact[i] = mk<double>(cf[i].re, cf[i].im);
+        } else {
+            SHHD_HIP(&err, hipMemcpy(compact.data(), h->flux, compact.size() * sizeof(cx<double>), hipMemcpyDeviceToHost));
+        }
+        compact_to_reference(*h, compact.data(), ns, out);
+        return SHHD_OK;
+    }
+
     template <typename T>
     StagePtrs<T> stage_ptrs(const Instance& h, int next, int cur, int base) const {
         StagePtrs<T> p{};
@@ -905,6 +969,8 @@ struct Engine {
             I.ubuf = static_cast<cx<T>*>(h.ubuf);
             I.flag = d_flags + lc.members[static_cast<size_t>(i)];
             I.step = d_steps + lc.members[static_cast<size_t>(i)];
+            I.flux = h.flux_req ? static_cast<cx<T>*>(h.flux) : nullptr;
+            I.flux_req = h.flux_req;
             I.C = h.cols.C;
             I.R = h.rows.R;
             I.ntiles = h.cols.ntiles;
@@ -1067,18 +1133,24 @@ struct Engine {
         for (int s = 0; s < 3; ++s) {
             for (auto& lc : classes) {
                 if (s >= lc.key.stages) continue;
+                // stage weights of the step flux: 1/6, 1/6, 2/3 (SRK3) or 1 (Euler)
+                static constexpr double FW[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
                 StageParams sp{};
                 if (lc.key.stages == 3) {
                     sp.beta = B[s];
                     sp.wb = WB[s];
                     sp.wc = WC[s];
                     sp.use_b = 1;
+                    sp.flux_w = FW[s];
                 } else {
                     sp.beta = 0.0;
                     sp.wb = 0.0;
                     sp.wc = 1.0;
                     sp.use_b = 0;
+                    sp.flux_w = 1.0;
                 }
+                sp.flux_first = s == 0 ? 1 : 0;
+                sp.flux_last = s == lc.key.stages - 1 ? 1 : 0;
                 shhd_status st = lc.key.fp32 ? launch_stage<float>(lc, k, s, sp, capturing)
                                              : launch_stage<double>(lc, k, s, sp, capturing);
                 if (st != SHHD_OK) return st;
@@ -1099,6 +1171,12 @@ struct Engine {
         if (dirty) {
             shhd_status st = rebuild();
             if (st != SHHD_OK) return st;
+        }
+        for (auto& sp : slots) {
+            if (sp && sp->flux && sp->flux_clear) {
+                SHHD_HIP(&err, hipMemsetAsync(sp->flux, 0, sp->flux_slots() * sp->field_elems * sp->elem, stream));
+                sp->flux_clear = false;
+            }
         }
         hipLaunchKernelGGL(k_clear_flags, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_flags, kMaxSlots);
         SHHD_HIP(&err, hipGetLastError());
@@ -1241,15 +1319,12 @@ shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0) {
 
 shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parameter, int momentum) {
     if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
-    if (density || order_parameter || momentum)
-        return set_err(&h->eng.err, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
-    return SHHD_OK;
+    return h->eng.set_flux_request(h->slot, density, order_parameter, momentum);
 }
 
 shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n) {
-    (void)out;
-    (void)n;
-    return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_UNSUPPORTED, "flux recording has no device path");
+    if (!h || !out) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return h->eng.get_flux(h->slot, out, n);
 }
 
 void shhd_gpu_destroy(shhd_gpu* h) { delete h; }
@@ -1336,6 +1411,16 @@ shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n)
 shhd_status shhd_batch_retire(shhd_batch* b, int slot) {
     if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
     return b->eng.retire(slot);
+}
+
+shhd_status shhd_batch_set_flux_request(shhd_batch* b, int slot, int density, int order_parameter, int momentum) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.set_flux_request(slot, density, order_parameter, momentum);
+}
+
+shhd_status shhd_batch_get_flux(shhd_batch* b, int slot, double* out, size_t n) {
+    if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return b->eng.get_flux(slot, out, n);
 }
 
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes) {

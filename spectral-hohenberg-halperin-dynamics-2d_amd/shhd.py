@@ -31,6 +31,7 @@ FREE_ENERGY_NULL, FREE_ENERGY_QUADRATIC, FREE_ENERGY_PHI4 = 0, 1, 2
 
 STATUS = {0: "SHHD_OK", 1: "SHHD_ERR_INVALID", 2: "SHHD_ERR_UNSUPPORTED", 3: "SHHD_ERR_HIP",
           4: "SHHD_ERR_ZERO_DENSITY", 5: "SHHD_ERR_RUNTIME", 6: "SHHD_ERR_FULL"}
+ERR_INVALID = 1
 ERR_FULL = 6
 
 
@@ -121,6 +122,10 @@ class SolverConfig:
 
     def state_shape(self):
         return (self.num_fields, self.ny, self.nx // 2 + 1)
+
+    def flux_shape(self):
+        """FluxBuffer slots (src/buffer_flux.cc:45-48): 2 + 2N + 3."""
+        return (2 + 2 * self.num_order_parameters + 3, self.ny, self.nx // 2 + 1)
 
     def to_c(self) -> ShhdConfig:
         c = ShhdConfig()
@@ -257,6 +262,10 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_batch_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.shhd_batch_set_flux_request.restype = ctypes.c_int
+    lib.shhd_batch_get_flux.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    lib.shhd_batch_get_flux.restype = ctypes.c_int
     lib.shhd_batch_get_state.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
     lib.shhd_batch_get_state.restype = ctypes.c_int
     lib.shhd_batch_retire.argtypes = [P, ctypes.c_int]
@@ -334,7 +343,17 @@ class GpuFluidSolver:
         self._check(self.lib.shhd_gpu_step(self.h, int(nsteps), float(t0)))
 
     def set_flux_request(self, density=False, order_parameter=False, momentum=False):
+        """FluidSolver::set_flux_request (src/fluid_solver.h:101-103)."""
         self._check(self.lib.shhd_gpu_set_flux_request(self.h, int(density), int(order_parameter), int(momentum)))
+
+    def get_flux(self) -> np.ndarray:
+        """FluidSolver::flux_buffer (src/fluid_solver.h:105-107): the last step's
+        fluxes, shape (2 + 2N + 3, My, Mx/2+1), slots [rho_x, rho_y,
+        (psi_x, psi_y) per order parameter, Pi_xx, Pi_xy, Pi_yy]."""
+        out = np.zeros(self.cfg.flux_shape(), dtype=np.complex128)
+        flat = out.view(np.float64).reshape(-1)
+        self._check(self.lib.shhd_gpu_get_flux(self.h, _dptr(flat), flat.size))
+        return out
 
     def info(self) -> ShhdInfo:
         i = ShhdInfo()
@@ -403,6 +422,17 @@ class Batch:
         out = np.zeros(cfg.state_shape(), dtype=np.complex128)
         flat = out.view(np.float64).reshape(-1)
         self._check(self.lib.shhd_batch_get_state(self.h, int(slot), _dptr(flat), flat.size))
+        return out
+
+    def set_flux_request(self, slot: int, density=False, order_parameter=False, momentum=False):
+        self._check(self.lib.shhd_batch_set_flux_request(self.h, int(slot), int(density), int(order_parameter),
+                                                         int(momentum)))
+
+    def get_flux(self, slot: int) -> np.ndarray:
+        cfg = self.cfgs[slot]
+        out = np.zeros(cfg.flux_shape(), dtype=np.complex128)
+        flat = out.view(np.float64).reshape(-1)
+        self._check(self.lib.shhd_batch_get_flux(self.h, int(slot), _dptr(flat), flat.size))
         return out
 
     def retire(self, slot: int):

@@ -54,3 +54,55 @@ def sine_momentum_state(cfg, amp, base_rho=1.0, nky=1):
     st[jx, nky, 0] = -0.5j * amp * m2
     st[jx, cfg.ny - nky, 0] = 0.5j * amp * m2
     return st
+
+
+def flux_divergence(cfg, flux):
+    """-i k . F of every conservation law, in state layout [rho, psi_a, jx, jy],
+    from the FluxBuffer slots [rho_x, rho_y, (psi_x, psi_y)_a, Pi_xx, Pi_xy, Pi_yy]
+    (src/buffer_flux.cc:10-48)."""
+    kx, ky, _ = shhd.wavenumbers(cfg)
+    n = cfg.num_order_parameters
+    out = np.zeros(cfg.state_shape(), complex)
+    out[0] = -1j * (kx * flux[0] + ky * flux[1])
+    for a in range(n):
+        out[1 + a] = -1j * (kx * flux[2 + 2 * a] + ky * flux[3 + 2 * a])
+    b = 2 + 2 * n
+    out[n + 1] = -1j * (kx * flux[b] + ky * flux[b + 1])
+    out[n + 2] = -1j * (kx * flux[b + 1] + ky * flux[b + 2])
+    return out
+
+
+def flux_configs():
+    """Configurations whose every right-hand-side term is the divergence of a
+    recorded flux (no sine or gradient forces), one per integrator and mode,
+    on a non-square grid with unequal lengths."""
+    base = dict(nx_active=32, ny_active=16, dealias="3/2", lx=20.0, ly=13.0)
+    phi4 = lambda n: dict(num_order_parameters=n, mobility=[1.0 + 0.3 * a for a in range(n)],  # noqa: E731
+                          free_energy_kind=shhd.FREE_ENERGY_PHI4, mu_k0=[-1.0] * n, mu_k2=[0.2] * n,
+                          mu_k4=[0.002] * n, phi4_u=[1.0] * n)
+    out = {}
+    for integ, tag in ((shhd.INTEGRATOR_SRK3, "srk3"), (shhd.INTEGRATOR_EULER, "euler")):
+        out[f"model_h_{tag}"] = shhd.SolverConfig(
+            **base, **phi4(2), integrator=integ, dt=0.004, pressure_coefficient=1.0, eta=0.5, zeta=0.3,
+            momentum_advection=True, order_parameter_advection=True, noise_momentum=True,
+            noise_order_parameter=True, kBT=1e-4)
+        out[f"incompressible_{tag}"] = shhd.SolverConfig(
+            **base, **phi4(1), integrator=integ, dt=0.004, mode=shhd.MODE_INCOMPRESSIBLE, eta=0.4,
+            momentum_advection=True, order_parameter_advection=True, noise_momentum=True,
+            noise_order_parameter=True, kBT=1e-4)
+        out[f"quiescent_{tag}"] = shhd.SolverConfig(
+            **base, **phi4(1), integrator=integ, dt=0.004, mode=shhd.MODE_QUIESCENT,
+            noise_order_parameter=True, kBT=0.02)
+    return out
+
+
+def flux_state(cfg, seed=3):
+    rng = np.random.default_rng(seed)
+    n = cfg.num_order_parameters
+    amp = {0: 0.01, n + 1: 0.02, n + 2: 0.02}
+    for a in range(n):
+        amp[1 + a] = 0.3
+    st = random_active_state(cfg, rng, amp=amp)
+    if n:
+        st[1, 0, 0] = 0.1 * cfg.nx * cfg.ny
+    return st
