@@ -159,6 +159,32 @@ shhd_status shhd_gpu_set_flux_request(shhd_gpu* h, int density, int order_parame
  * are not requested read 0. */
 shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n);
 
+/* ---- on-device FP64 accumulators -------------------------------------------
+ * Measures that read a few modes every step (TimeSeriesMeasure's flux columns,
+ * src/measure_time_series.cc:189-227; YokotaGreenKuboMeasure's Pi_xy at chosen
+ * wave vectors, src/measure_yokota_green_kubo.cc:128-193) or a per-mode average
+ * (structure factor) are fed on the device, with no per-step download. */
+typedef struct shhd_probe {
+    int source;            /* 0: state field, 1: flux slot (see shhd_gpu_get_flux) */
+    int index;             /* field or slot index                              */
+    int gx, gy;            /* reference-layout mode, inside the active mask    */
+} shhd_probe;
+
+/* After every step, append the values of the probed modes (as FP64 complex)
+ * to a ring of `capacity` rows.  nprobes = 0 removes the probes.  A step that
+ * would overflow the ring fails with SHHD_ERR_FULL before doing any work. */
+shhd_status shhd_gpu_set_probes(shhd_gpu* h, const shhd_probe* probes, int nprobes, int capacity);
+/* Copy the rows recorded since the last read, oldest first, into out
+ * (rows * nprobes * 2 doubles, n >= that) and empty the ring. */
+shhd_status shhd_gpu_read_probes(shhd_gpu* h, double* out, size_t n, int* nrows);
+/* Every `nevery` steps from now on add |f_hat(k)|^2 of each field in
+ * field_mask (bit f = state field f) to an FP64 sum per active mode.
+ * field_mask = 0 removes it; a new call restarts the sums. */
+shhd_status shhd_gpu_set_structure_factor(shhd_gpu* h, unsigned field_mask, int nevery);
+/* The sums, one real[(Mx/2+1) * My] per selected field in field order
+ * (reference layout, inactive modes 0), and the number of samples. */
+shhd_status shhd_gpu_get_structure_factor(shhd_gpu* h, double* out, size_t n, long long* nsamples);
+
 /* Destroy the handle and free its device memory. */
 void shhd_gpu_destroy(shhd_gpu* h);
 
@@ -208,6 +234,11 @@ shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n)
 /* flux recording of one instance, as shhd_gpu_set_flux_request / shhd_gpu_get_flux */
 shhd_status shhd_batch_set_flux_request(shhd_batch* b, int slot, int density, int order_parameter, int momentum);
 shhd_status shhd_batch_get_flux(shhd_batch* b, int slot, double* out, size_t n);
+/* on-device accumulators of one instance, as the shhd_gpu_* versions */
+shhd_status shhd_batch_set_probes(shhd_batch* b, int slot, const shhd_probe* probes, int nprobes, int capacity);
+shhd_status shhd_batch_read_probes(shhd_batch* b, int slot, double* out, size_t n, int* nrows);
+shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned field_mask, int nevery);
+shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples);
 /* free an instance's device memory and slot */
 shhd_status shhd_batch_retire(shhd_batch* b, int slot);
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes);

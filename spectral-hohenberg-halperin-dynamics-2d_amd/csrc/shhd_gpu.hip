@@ -30,6 +30,7 @@
 
 #include "../../include/shhd_gpu.h"
 #include "instances.h"
+#include "measure.h"
 
 using namespace shhd;
 
@@ -272,8 +273,43 @@ struct Instance {
     int flux_req = 0;           // FluxBits
     bool flux_clear = false;    // zero every slot before the next step
     int flux_slots() const { return 2 + 2 * desc.N + 3; }
+    // on-device accumulators (measure.h)
+    std::vector<Probe> probes;
+    Probe* d_probes = nullptr;
+    cx<double>* d_ring = nullptr;
+    int* d_rows = nullptr;
+    int capacity = 0;
+    int probe_rows = 0;         // rows recorded since the last read (host count)
+    size_t probe_bytes = 0;
+    std::vector<int> sk_fields;
+    int sk_nevery = 0;
+    uint64_t sk_base = 0;
+    double* d_sk = nullptr;
+    unsigned long long* d_sk_count = nullptr;
+    size_t sk_bytes = 0;
+
+    void free_probes() {
+        if (d_probes) (void)hipFree(d_probes);
+        if (d_ring) (void)hipFree(d_ring);
+        if (d_rows) (void)hipFree(d_rows);
+        d_probes = nullptr;
+        d_ring = nullptr;
+        d_rows = nullptr;
+        probes.clear();
+        capacity = probe_rows = 0;
+    }
+    void free_sk() {
+        if (d_sk) (void)hipFree(d_sk);
+        if (d_sk_count) (void)hipFree(d_sk_count);
+        d_sk = nullptr;
+        d_sk_count = nullptr;
+        sk_fields.clear();
+        sk_nevery = 0;
+    }
 
     ~Instance() {
+        free_probes();
+        free_sk();
         if (flux) (void)hipFree(flux);
         for (int b = 0; b < 3; ++b) {
             if (state[b]) (void)hipFree(state[b]);
@@ -758,6 +794,9 @@ struct LaunchClass {
     int* d_map = nullptr;            // block maps of K1, K2, K3
     void* d_static = nullptr;        // StaticEntry<T>[nstatic]
     int nstatic = 0;
+    void* d_meas = nullptr;          // MeasDev<T>[n] when any member measures
+    bool has_probe = false, has_sk = false;
+    int sk_blocks = 0;
     int nblk[3] = {0, 0, 0};
     BlockMap map[3] = {};
     size_t lds[3] = {0, 0, 0};
@@ -767,7 +806,8 @@ struct LaunchClass {
         if (d_ptrs) (void)hipFree(d_ptrs);
         if (d_map) (void)hipFree(d_map);
         if (d_static) (void)hipFree(d_static);
-        d_inst = d_ptrs = d_static = nullptr;
+        if (d_meas) (void)hipFree(d_meas);
+        d_inst = d_ptrs = d_static = d_meas = nullptr;
         d_map = nullptr;
     }
 };
@@ -905,6 +945,141 @@ struct Engine {
         return SHHD_OK;
     }
 
+    // Mode probes: after every step the values of the chosen (state field or
+    // flux slot, reference mode) pairs are appended to a ring of `capacity`
+    // FP64 rows.  nprobes = 0 removes them.
+    shhd_status set_probes(int slot, const shhd_probe* probes, int nprobes, int capacity) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (nprobes < 0 || nprobes > kMaxProbes)
+            return set_err(&err, SHHD_ERR_INVALID, "number of probes must be in [0, " + std::to_string(kMaxProbes) + "]");
+        if (nprobes > 0 && (!probes || capacity <= 0))
+            return set_err(&err, SHHD_ERR_INVALID, "probes need a list and a positive capacity");
+        const shhd_config& c = h->cfg;
+        std::vector<Probe> list;
+        for (int p = 0; p < nprobes; ++p) {
+            const shhd_probe& q = probes[p];
+            const int limit = q.source == 0 ? h->desc.nfields : h->flux_slots();
+            if ((q.source != 0 && q.source != 1) || q.index < 0 || q.index >= limit)
+                return set_err(&err, SHHD_ERR_INVALID, "probe " + std::to_string(p) + ": invalid source or index");
+            if (q.gx < 0 || q.gx >= c.nx / 2 + 1 || q.gy < 0 || q.gy >= c.ny || !ref_active(c, q.gx, q.gy))
+                return set_err(&err, SHHD_ERR_INVALID, "probe " + std::to_string(p) + ": mode is outside the active spectral mask");
+            const int ky = (q.gy <= c.ny / 2) ? q.gy : q.gy - c.ny;
+            const int cidx = ky >= 0 ? ky : ky + c.ny_active;
+            list.push_back({q.source, q.index, q.gx * c.ny_active + cidx});
+        }
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        used -= h->probe_bytes;
+        h->bytes -= h->probe_bytes;
+        h->probe_bytes = 0;
+        h->free_probes();
+        dirty = true;
+        if (nprobes == 0) return SHHD_OK;
+        const size_t ring = static_cast<size_t>(capacity) * static_cast<size_t>(nprobes) * sizeof(cx<double>);
+        const size_t pb = ring + list.size() * sizeof(Probe) + sizeof(int);
+        if (budget && used + pb > budget)
+            return set_err(&err, SHHD_ERR_FULL, "device budget exhausted: probe ring needs " + std::to_string(pb) + " bytes");
+        SHHD_HIP(&err, hipMalloc(&h->d_probes, list.size() * sizeof(Probe)));
+        SHHD_HIP(&err, hipMemcpy(h->d_probes, list.data(), list.size() * sizeof(Probe), hipMemcpyHostToDevice));
+        SHHD_HIP(&err, hipMalloc(&h->d_ring, ring));
+        SHHD_HIP(&err, hipMalloc(&h->d_rows, sizeof(int)));
+        SHHD_HIP(&err, hipMemset(h->d_rows, 0, sizeof(int)));
+        h->probes = list;
+        h->capacity = capacity;
+        h->probe_rows = 0;
+        h->probe_bytes = pb;
+        h->bytes += pb;
+        used += pb;
+        return SHHD_OK;
+    }
+
+    // The rows recorded since the last read, oldest first, as
+    // rows x nprobes FP64 complex; the ring is emptied.
+    shhd_status read_probes(int slot, double* out, size_t n, int* nrows) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (h->probes.empty()) return set_err(&err, SHHD_ERR_INVALID, "no probes are set");
+        const size_t need = 2 * static_cast<size_t>(h->probe_rows) * h->probes.size();
+        if (n < need) return set_err(&err, SHHD_ERR_INVALID, "probe output too small: " + std::to_string(need) + " doubles needed");
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        int rows = 0;
+        SHHD_HIP(&err, hipMemcpy(&rows, h->d_rows, sizeof(int), hipMemcpyDeviceToHost));
+        if (rows != h->probe_rows) return set_err(&err, SHHD_ERR_RUNTIME, "probe ring row count mismatch");
+        if (need) SHHD_HIP(&err, hipMemcpy(out, h->d_ring, need * sizeof(double), hipMemcpyDeviceToHost));
+        SHHD_HIP(&err, hipMemset(h->d_rows, 0, sizeof(int)));
+        if (nrows) *nrows = rows;
+        h->probe_rows = 0;
+        return SHHD_OK;
+    }
+
+    // Structure factor: every `nevery` steps from now on, |f_hat(k)|^2 of each
+    // field in field_mask (bit f = state field f) is added to an FP64 sum per
+    // active mode.  field_mask = 0 removes it; calling again restarts it.
+    shhd_status set_structure_factor(int slot, unsigned field_mask, int nevery) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (field_mask >> h->desc.nfields)
+            return set_err(&err, SHHD_ERR_INVALID, "structure-factor field mask names a field the state does not have");
+        if (field_mask && nevery <= 0) return set_err(&err, SHHD_ERR_INVALID, "structure-factor interval must be positive");
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        used -= h->sk_bytes;
+        h->bytes -= h->sk_bytes;
+        h->sk_bytes = 0;
+        h->free_sk();
+        dirty = true;
+        if (!field_mask) return SHHD_OK;
+        std::vector<int> fields;
+        for (int f = 0; f < h->desc.nfields; ++f)
+            if ((field_mask >> f) & 1u) fields.push_back(f);
+        const size_t sb = fields.size() * h->field_elems * sizeof(double) + sizeof(unsigned long long);
+        if (budget && used + sb > budget)
+            return set_err(&err, SHHD_ERR_FULL, "device budget exhausted: structure factor needs " + std::to_string(sb) + " bytes");
+        SHHD_HIP(&err, hipMalloc(&h->d_sk, fields.size() * h->field_elems * sizeof(double)));
+        SHHD_HIP(&err, hipMemset(h->d_sk, 0, fields.size() * h->field_elems * sizeof(double)));
+        SHHD_HIP(&err, hipMalloc(&h->d_sk_count, sizeof(unsigned long long)));
+        SHHD_HIP(&err, hipMemset(h->d_sk_count, 0, sizeof(unsigned long long)));
+        h->sk_fields = fields;
+        h->sk_nevery = nevery;
+        h->sk_base = h->step_index;
+        h->sk_bytes = sb;
+        h->bytes += sb;
+        used += sb;
+        return SHHD_OK;
+    }
+
+    // The sums in the reference layout, one real[(Mx/2+1) * My] per selected
+    // field in field order (inactive modes 0), and the number of samples.
+    shhd_status get_structure_factor(int slot, double* out, size_t n, long long* nsamples) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (h->sk_fields.empty()) return set_err(&err, SHHD_ERR_INVALID, "no structure factor is accumulated");
+        const shhd_config& c = h->cfg;
+        const size_t spec = ref_spec(c);
+        const size_t nf = h->sk_fields.size();
+        if (n != nf * spec) return set_err(&err, SHHD_ERR_INVALID, "structure-factor size mismatch");
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        std::vector<double> compact(nf * h->field_elems);
+        SHHD_HIP(&err, hipMemcpy(compact.data(), h->d_sk, compact.size() * sizeof(double), hipMemcpyDeviceToHost));
+        unsigned long long cnt = 0;
+        SHHD_HIP(&err, hipMemcpy(&cnt, h->d_sk_count, sizeof(cnt), hipMemcpyDeviceToHost));
+        std::memset(out, 0, sizeof(double) * n);
+        const int hx = c.nx / 2 + 1;
+        const int nay = c.ny_active;
+        for (size_t q = 0; q < nf; ++q) {
+            for (int gy = 0; gy < c.ny; ++gy) {
+                for (int gx = 0; gx < c.nx_active / 2; ++gx) {
+                    if (!ref_active(c, gx, gy)) continue;
+                    const int ky = (gy <= c.ny / 2) ? gy : gy - c.ny;
+                    const int cidx = ky >= 0 ? ky : ky + nay;
+                    out[q * spec + static_cast<size_t>(gy) * hx + gx] =
+                        compact[q * h->field_elems + static_cast<size_t>(gx) * nay + cidx];
+                }
+            }
+        }
+        if (nsamples) *nsamples = static_cast<long long>(cnt);
+        return SHHD_OK;
+    }
+
     // FluidSolver::flux_buffer (src/fluid_solver.h:105-107) in the reference
     // layout: slots of complex[(Mx/2+1) * My], inactive modes 0.
     shhd_status get_flux(int slot, double* out, size_t n) {
@@ -1014,6 +1189,43 @@ struct Engine {
             lc.map[k].first = lc.d_map + off;
             SHHD_HIP(&err, hipMemcpy(lc.d_map + off, firsts[k].data(), firsts[k].size() * sizeof(int), hipMemcpyHostToDevice));
             off += firsts[k].size();
+        }
+        // on-device accumulators
+        std::vector<MeasDev<T>> meas(static_cast<size_t>(n));
+        lc.has_probe = lc.has_sk = false;
+        lc.sk_blocks = 0;
+        for (int i = 0; i < n; ++i) {
+            const int slot = lc.members[static_cast<size_t>(i)];
+            Instance& h = *slots[static_cast<size_t>(slot)];
+            MeasDev<T>& M = meas[static_cast<size_t>(i)];
+            M = MeasDev<T>{};
+            M.nprobe = static_cast<int>(h.probes.size());
+            M.capacity = h.capacity;
+            M.probes = h.d_probes;
+            M.ring = h.d_ring;
+            M.rows = h.d_rows;
+            M.flux = static_cast<const cx<T>*>(h.flux);
+            M.sk_nf = static_cast<int>(h.sk_fields.size());
+            for (int q = 0; q < M.sk_nf; ++q) M.sk_field[q] = h.sk_fields[static_cast<size_t>(q)];
+            M.sk_nevery = std::max(1, h.sk_nevery);
+            M.sk_base = h.sk_base;
+            M.sk = h.d_sk;
+            M.sk_count = h.d_sk_count;
+            M.step = d_steps + slot;
+            M.field_elems = h.field_elems;
+            M.static_mask = 0;
+            for (int f : h.static_fields) M.static_mask |= 1u << f;
+            M.k0_static = h.fp32 ? h.k0[0] : nullptr;
+            lc.has_probe = lc.has_probe || M.nprobe > 0;
+            if (M.sk_nf > 0) {
+                lc.has_sk = true;
+                const size_t total = static_cast<size_t>(M.sk_nf) * h.field_elems;
+                lc.sk_blocks = std::max(lc.sk_blocks, static_cast<int>(std::min<size_t>(1024, (total + 255) / 256)));
+            }
+        }
+        if (lc.has_probe || lc.has_sk) {
+            SHHD_HIP(&err, hipMalloc(&lc.d_meas, meas.size() * sizeof(MeasDev<T>)));
+            SHHD_HIP(&err, hipMemcpy(lc.d_meas, meas.data(), meas.size() * sizeof(MeasDev<T>), hipMemcpyHostToDevice));
         }
         lc.nstatic = static_cast<int>(statics.size());
         if (lc.nstatic) {
@@ -1156,7 +1368,26 @@ struct Engine {
                 if (st != SHHD_OK) return st;
             }
         }
+        for (auto& lc : classes) {
+            if (!lc.d_meas) continue;
+            shhd_status st = lc.key.fp32 ? launch_measures<float>(lc, k) : launch_measures<double>(lc, k);
+            if (st != SHHD_OK) return st;
+        }
         hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, kMaxSlots);
+        SHHD_HIP(&err, hipGetLastError());
+        return SHHD_OK;
+    }
+
+    // probes and structure factor on the state the step just wrote
+    template <typename T>
+    shhd_status launch_measures(LaunchClass& lc, int k) {
+        const int n = static_cast<int>(lc.members.size());
+        const MeasDev<T>* meas = static_cast<const MeasDev<T>*>(lc.d_meas);
+        const StagePtrs<T>* last =
+            static_cast<const StagePtrs<T>*>(lc.d_ptrs) + static_cast<size_t>(k * lc.key.stages + lc.key.stages - 1) * n;
+        if (lc.has_probe) hipLaunchKernelGGL(k_probe<T>, dim3(n), dim3(256), 0, stream, meas, last);
+        if (lc.has_sk)
+            hipLaunchKernelGGL(k_structure_factor<T>, dim3(lc.sk_blocks, n), dim3(256), 0, stream, meas, last);
         SHHD_HIP(&err, hipGetLastError());
         return SHHD_OK;
     }
@@ -1167,6 +1398,12 @@ struct Engine {
         for (auto& sp : slots) {
             if (sp && sp->rho0_needed && sp->desc.rho0 == 0.0)
                 return set_err(&err, SHHD_ERR_RUNTIME, "reference density must be nonzero.");
+        }
+        for (auto& sp : slots) {
+            if (sp && sp->capacity && sp->probe_rows + nsteps > sp->capacity)
+                return set_err(&err, SHHD_ERR_FULL, "probe ring would overflow: " + std::to_string(sp->probe_rows) +
+                                                         " of " + std::to_string(sp->capacity) +
+                                                         " rows recorded; read the probes first");
         }
         if (dirty) {
             shhd_status st = rebuild();
@@ -1206,6 +1443,7 @@ struct Engine {
             if (!sp) continue;
             sp->cur = (sp->cur + nsteps) % 3;
             sp->step_index += static_cast<uint64_t>(nsteps);
+            if (sp->capacity) sp->probe_rows += nsteps;
         }
         if (nsteps % 3 != 0) dirty = true;   // stage-pointer tables are built for the current rotation
         std::vector<int> flags(kMaxSlots);
@@ -1327,6 +1565,26 @@ shhd_status shhd_gpu_get_flux(shhd_gpu* h, double* out, size_t n) {
     return h->eng.get_flux(h->slot, out, n);
 }
 
+shhd_status shhd_gpu_set_probes(shhd_gpu* h, const shhd_probe* probes, int nprobes, int capacity) {
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return h->eng.set_probes(h->slot, probes, nprobes, capacity);
+}
+
+shhd_status shhd_gpu_read_probes(shhd_gpu* h, double* out, size_t n, int* nrows) {
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return h->eng.read_probes(h->slot, out, n, nrows);
+}
+
+shhd_status shhd_gpu_set_structure_factor(shhd_gpu* h, unsigned field_mask, int nevery) {
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return h->eng.set_structure_factor(h->slot, field_mask, nevery);
+}
+
+shhd_status shhd_gpu_get_structure_factor(shhd_gpu* h, double* out, size_t n, long long* nsamples) {
+    if (!h || !out) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return h->eng.get_structure_factor(h->slot, out, n, nsamples);
+}
+
 void shhd_gpu_destroy(shhd_gpu* h) { delete h; }
 
 const char* shhd_gpu_last_error(const shhd_gpu* h) { return h ? h->eng.err.c_str() : g_create_error.c_str(); }
@@ -1421,6 +1679,26 @@ shhd_status shhd_batch_set_flux_request(shhd_batch* b, int slot, int density, in
 shhd_status shhd_batch_get_flux(shhd_batch* b, int slot, double* out, size_t n) {
     if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
     return b->eng.get_flux(slot, out, n);
+}
+
+shhd_status shhd_batch_set_probes(shhd_batch* b, int slot, const shhd_probe* probes, int nprobes, int capacity) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.set_probes(slot, probes, nprobes, capacity);
+}
+
+shhd_status shhd_batch_read_probes(shhd_batch* b, int slot, double* out, size_t n, int* nrows) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.read_probes(slot, out, n, nrows);
+}
+
+shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned field_mask, int nevery) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.set_structure_factor(slot, field_mask, nevery);
+}
+
+shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples) {
+    if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return b->eng.get_structure_factor(slot, out, n, nsamples);
 }
 
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes) {

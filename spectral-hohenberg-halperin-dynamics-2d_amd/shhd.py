@@ -40,6 +40,13 @@ class ShhdSineForce(ctypes.Structure):
                 ("nk", ctypes.c_int), ("amplitude", ctypes.c_double)]
 
 
+class ShhdProbe(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int), ("index", ctypes.c_int), ("gx", ctypes.c_int), ("gy", ctypes.c_int)]
+
+
+PROBE_STATE, PROBE_FLUX = 0, 1
+
+
 class ShhdGradientForce(ctypes.Structure):
     _fields_ = [("component", ctypes.c_int), ("direction", ctypes.c_int), ("amplitude", ctypes.c_double)]
 
@@ -262,6 +269,19 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    for pre, first in (("shhd_gpu_", [P]), ("shhd_batch_", [P, ctypes.c_int])):
+        f = getattr(lib, pre + "set_probes")
+        f.argtypes = first + [ctypes.POINTER(ShhdProbe), ctypes.c_int, ctypes.c_int]
+        f.restype = ctypes.c_int
+        f = getattr(lib, pre + "read_probes")
+        f.argtypes = first + [ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        f.restype = ctypes.c_int
+        f = getattr(lib, pre + "set_structure_factor")
+        f.argtypes = first + [ctypes.c_uint, ctypes.c_int]
+        f.restype = ctypes.c_int
+        f = getattr(lib, pre + "get_structure_factor")
+        f.argtypes = first + [ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]
+        f.restype = ctypes.c_int
     lib.shhd_batch_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.shhd_batch_set_flux_request.restype = ctypes.c_int
     lib.shhd_batch_get_flux.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
@@ -290,7 +310,51 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-class GpuFluidSolver:
+def _probe_array(probes):
+    arr = (ShhdProbe * max(1, len(probes)))()
+    for i, (src, idx, gx, gy) in enumerate(probes):
+        arr[i] = ShhdProbe(int(src), int(idx), int(gx), int(gy))
+    return arr
+
+
+def _field_mask(fields):
+    m = 0
+    for f in fields:
+        m |= 1 << int(f)
+    return m
+
+
+class _Measures:
+    """Probes and structure factor (include/shhd_gpu.h, on-device FP64
+    accumulators); `_call(name, *args)` adds the handle (and slot)."""
+
+    def _set_probes(self, key, probes, capacity):
+        self._probes[key] = len(probes)
+        self._check(self._call("set_probes", key, _probe_array(probes), len(probes), int(capacity)))
+        self._cap[key] = int(capacity)
+
+    def _read_probes(self, key):
+        n = self._probes.get(key, 0)
+        out = np.zeros((self._cap.get(key, 0), n), dtype=np.complex128)
+        flat = out.view(np.float64).reshape(-1)
+        rows = ctypes.c_int()
+        self._check(self._call("read_probes", key, _dptr(flat), flat.size, ctypes.byref(rows)))
+        return out[:rows.value].copy()
+
+    def _set_sk(self, key, fields, nevery):
+        self._sk[key] = sorted(int(f) for f in fields)
+        self._check(self._call("set_structure_factor", key, _field_mask(fields), int(nevery)))
+
+    def _get_sk(self, key, cfg):
+        fields = self._sk.get(key, [])
+        out = np.zeros((len(fields), cfg.ny, cfg.nx // 2 + 1), dtype=np.float64)
+        cnt = ctypes.c_longlong()
+        flat = out.reshape(-1)
+        self._check(self._call("get_structure_factor", key, _dptr(flat), flat.size, ctypes.byref(cnt)))
+        return out, cnt.value
+
+
+class GpuFluidSolver(_Measures):
     """Drop-in for the reference FluidSolver's step path on one HIP device."""
 
     KERNEL_CLASSES = ("inverse_columns", "rows", "forward_columns_stage")
@@ -304,10 +368,29 @@ class GpuFluidSolver:
         if st != 0:
             raise ShhdError(st, self.lib.shhd_gpu_last_error(None).decode())
         self.h = h
+        self._probes, self._cap, self._sk = {}, {}, {}
 
     def _check(self, st):
         if st != 0:
             raise ShhdError(st, self.lib.shhd_gpu_last_error(self.h).decode())
+
+    def _call(self, name, key, *args):
+        return getattr(self.lib, "shhd_gpu_" + name)(self.h, *args)
+
+    def set_probes(self, probes, capacity):
+        """probes: list of (source, index, gx, gy); source PROBE_STATE or PROBE_FLUX."""
+        self._set_probes(0, probes, capacity)
+
+    def read_probes(self) -> np.ndarray:
+        """Rows recorded since the last read: shape (rows, nprobes), complex."""
+        return self._read_probes(0)
+
+    def set_structure_factor(self, fields, nevery=1):
+        self._set_sk(0, fields, nevery)
+
+    def get_structure_factor(self):
+        """(sums of |f_hat|^2, shape (nfields, My, Mx/2+1)), number of samples)."""
+        return self._get_sk(0, self.cfg)
 
     def close(self):
         if getattr(self, "h", None):
@@ -375,7 +458,7 @@ class GpuFluidSolver:
         return {k: b[i] for i, k in enumerate(self.KERNEL_CLASSES)}
 
 
-class Batch:
+class Batch(_Measures):
     """Many resident simulations advanced together (include/shhd_gpu.h,
     shhd_batch_*).  Each admitted instance computes exactly what it would
     compute alone."""
@@ -388,10 +471,26 @@ class Batch:
             raise ShhdError(st, self.lib.shhd_batch_last_error(None).decode())
         self.h = h
         self.cfgs = {}
+        self._probes, self._cap, self._sk = {}, {}, {}
 
     def _check(self, st):
         if st != 0:
             raise ShhdError(st, self.lib.shhd_batch_last_error(self.h).decode())
+
+    def _call(self, name, key, *args):
+        return getattr(self.lib, "shhd_batch_" + name)(self.h, int(key), *args)
+
+    def set_probes(self, slot, probes, capacity):
+        self._set_probes(slot, probes, capacity)
+
+    def read_probes(self, slot) -> np.ndarray:
+        return self._read_probes(slot)
+
+    def set_structure_factor(self, slot, fields, nevery=1):
+        self._set_sk(slot, fields, nevery)
+
+    def get_structure_factor(self, slot):
+        return self._get_sk(slot, self.cfgs[slot])
 
     @staticmethod
     def footprint(cfg: SolverConfig) -> int:
