@@ -185,6 +185,53 @@ shhd_status shhd_gpu_set_structure_factor(shhd_gpu* h, unsigned field_mask, int 
  * (reference layout, inactive modes 0), and the number of samples. */
 shhd_status shhd_gpu_get_structure_factor(shhd_gpu* h, double* out, size_t n, long long* nsamples);
 
+/* ---- device-side initial conditions -----------------------------------------
+ * `set <target> <style> ...` (src/initial_condition_*.cc) evaluated on the
+ * device straight into the resident state: the mode-hash styles are pure
+ * functions of (seed, channel, gx, ky, draw) through splitmix64, so a large
+ * grid needs no host generation and no upload.  Like the reference's apply(),
+ * each call overwrites one field (every mode; only active modes nonzero). */
+enum {
+    /* set density equilibrium/gaussian rho0 kBT cT seed
+       (src/initial_condition_equilibrium_gaussian_density.cc:56-93) */
+    SHHD_IC_EQUILIBRIUM_DENSITY = 0,
+    /* set order_parameter <a> equilibrium/gaussian psi0 kBT k0 k2 k4 seed
+       (src/initial_condition_equilibrium_gaussian_order_parameter.cc:59-108) */
+    SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER = 1,
+    /* set momentum <x|y> equilibrium/gaussian/compressible kBT seed
+       (src/initial_condition_equilibrium_gaussian_momentum_compressible.cc:76-112) */
+    SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE = 2,
+    /* set momentum <x|y> equilibrium/gaussian/incompressible kBT seed
+       (src/initial_condition_equilibrium_gaussian_momentum_incompressible.cc:76-115) */
+    SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE = 3,
+    /* set momentum <x|y> random_vorticity n0 sigma urms seed
+       (src/initial_condition_random_vorticity_momentum.cc:101-147) */
+    SHHD_IC_RANDOM_VORTICITY = 4,
+    /* No reference counterpart: transverse random-phase velocity
+       u_hat = i (-ky, kx)/|k| n^(3/2) exp(-(n/n0)^2) e^{i theta} over integer
+       wave numbers n = |(gx, ky)| (energy spectrum E(n) ∝ n^4 exp(-2 (n/n0)^2)),
+       phases hashed as random_vorticity's, normalized like it to
+       <|u|^2> = urms^2, times rho0 (the incompressible-turbulence start
+       E(k) ∝ k^4 exp(-2(k/8)^2)). */
+    SHHD_IC_SPECTRUM_K4 = 5
+};
+
+typedef struct shhd_initial_condition {
+    int kind;                 /* SHHD_IC_*                                        */
+    int component;            /* order-parameter index, or momentum direction 0/1  */
+    int seed;
+    double kBT;               /* equilibrium styles                              */
+    double value;             /* rho0 (density) or psi0 (order parameter)        */
+    double pressure_coefficient; /* density: cT^2                                */
+    double k0, k2, k4;        /* order parameter: linear chemical-potential coefficients */
+    double n0, sigma, urms;   /* random_vorticity (n0, sigma, urms); spectrum (n0, urms) */
+} shhd_initial_condition;
+
+/* Momentum styles read rho0 = Re rho_hat(0,0) / (Mx*My) from the resident
+ * state, as the reference does, so density is set first.  Errors carry the
+ * reference's messages. */
+shhd_status shhd_gpu_apply_initial_condition(shhd_gpu* h, const shhd_initial_condition* ic);
+
 /* Destroy the handle and free its device memory. */
 void shhd_gpu_destroy(shhd_gpu* h);
 
@@ -239,6 +286,8 @@ shhd_status shhd_batch_set_probes(shhd_batch* b, int slot, const shhd_probe* pro
 shhd_status shhd_batch_read_probes(shhd_batch* b, int slot, double* out, size_t n, int* nrows);
 shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned field_mask, int nevery);
 shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples);
+/* initial condition of one instance, as shhd_gpu_apply_initial_condition */
+shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic);
 /* free an instance's device memory and slot */
 shhd_status shhd_batch_retire(shhd_batch* b, int slot);
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes);

@@ -209,6 +209,49 @@ void fft2_c2r_many(int mx, int my, int nfields, const cplx* spec, double* phys, 
 // -------------------------------------------------------------------------
 // Spectral mask (src/spectral_mask.cc:7-72) on a single-rank domain.
 // -------------------------------------------------------------------------
+// -------------------------------------------------------------------------
+// Mode-hash random numbers of the initial conditions
+// (src/initial_condition_equilibrium_gaussian_density.cc:12-50,
+//  src/initial_condition_random_vorticity_momentum.cc:12-34)
+// -------------------------------------------------------------------------
+uint64_t ic_splitmix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+double ic_uniform(int seed, int channel, int gx, int ky, int draw) {
+    uint64_t key = static_cast<uint64_t>(static_cast<uint32_t>(seed));
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(channel)) << 48;
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(gx)) << 32;
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(ky)) << 8;
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(draw));
+    return static_cast<double>(ic_splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// gaussian_mode: Box-Muller of two hashed uniforms; the kx = 0 line hashes
+// |ky| and conjugates ky < 0
+cplx ic_gaussian(int seed, int channel, int gx, int ky) {
+    const int phase_ky = gx == 0 ? std::abs(ky) : ky;
+    const double u1 = std::max(ic_uniform(seed, channel, gx, phase_ky, 0), 1.0e-300);
+    const double u2 = ic_uniform(seed, channel, gx, phase_ky, 1);
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    const double th = 2.0 * kPi * u2;
+    cplx v(r * std::cos(th), r * std::sin(th));
+    return (gx == 0 && ky < 0) ? std::conj(v) : v;
+}
+
+// random_phase of random_vorticity: one hashed uniform without channel/draw
+cplx ic_phase(int seed, int gx, int ky) {
+    uint64_t key = static_cast<uint64_t>(static_cast<uint32_t>(seed));
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(gx)) << 32;
+    key ^= static_cast<uint64_t>(static_cast<uint32_t>(ky));
+    const double u = static_cast<double>(ic_splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+    const double th = 2.0 * kPi * u;
+    return cplx(std::cos(th), std::sin(th));
+}
+
 struct Mode {
     size_t index;
     int gx, gy;
@@ -835,6 +878,116 @@ struct Oracle {
     }
 
     // ---------------------------------------------------------------------
+    // initial conditions (src/initial_condition_*.cc)
+    // ---------------------------------------------------------------------
+    int signed_ky(int gy) const { return gy <= my / 2 ? gy : gy - my; }
+
+    double state_rho0() const { return state[0].real() / (static_cast<double>(mx) * static_cast<double>(my)); }
+
+    // velocity of the random-vorticity / spectrum styles before normalization
+    void ic_velocity(const shhd_initial_condition& ic, const Mode& m, cplx& ux, cplx& uy) const {
+        ux = uy = cplx(0, 0);
+        if (m.k2 == 0.0) return;
+        const int ky = signed_ky(m.gy);
+        const int phase_ky = m.gx == 0 ? std::abs(ky) : ky;
+        const double n = std::sqrt(static_cast<double>(m.gx) * m.gx + static_cast<double>(ky) * ky);
+        cplx ph = ic_phase(ic.seed, m.gx, phase_ky);
+        if (m.gx == 0 && ky < 0) ph = std::conj(ph);
+        if (ic.kind == SHHD_IC_RANDOM_VORTICITY) {
+            // src/initial_condition_random_vorticity_momentum.cc:56-99
+            const double z = (n - ic.n0) / ic.sigma;
+            const cplx omega = std::exp(-0.5 * z * z) * ph;
+            ux = cplx(0.0, m.ky / m.k2) * omega;
+            uy = -cplx(0.0, m.kx / m.k2) * omega;
+        } else {
+            // i (-ky, kx)/|k| times n^(3/2) exp(-(n/n0)^2): transverse, and
+            // Hermitian on the kx = 0 line (the factor i makes the odd
+            // component real in physical space)
+            const double a = std::pow(n, 1.5) * std::exp(-(n / ic.n0) * (n / ic.n0));
+            const double k = std::sqrt(m.k2);
+            ux = I * ((-m.ky / k) * a * ph);
+            uy = I * ((m.kx / k) * a * ph);
+        }
+    }
+
+    void apply_ic(const shhd_initial_condition& ic) {
+        const double grid = static_cast<double>(mx) * static_cast<double>(my);
+        const double cell = cfg.lx * cfg.ly / grid;
+        int f = -1;
+        switch (ic.kind) {
+            case SHHD_IC_EQUILIBRIUM_DENSITY: f = 0; break;
+            case SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER:
+                if (ic.component < 0 || ic.component >= N) throw std::runtime_error("order-parameter index out of range");
+                f = psi_field(ic.component);
+                break;
+            default:
+                if (ic.component != 0 && ic.component != 1) throw std::runtime_error("momentum direction must be 0 or 1");
+                f = jx_field() + ic.component;
+        }
+        cplx* out = state.data() + static_cast<size_t>(f) * spec;
+        if (ic.kind == SHHD_IC_EQUILIBRIUM_DENSITY) {   // :69-93
+            std::fill(out, out + spec, cplx(0, 0));
+            out[0] = cplx(ic.value * grid, 0.0);
+            if (ic.kBT == 0.0) return;
+            const double sigma = std::sqrt(ic.kBT * grid * ic.value / (2.0 * ic.pressure_coefficient * cell));
+            for (const Mode& m : active)
+                if (m.k2 != 0.0) out[m.index] = sigma * ic_gaussian(ic.seed, 0, m.gx, signed_ky(m.gy));
+        } else if (ic.kind == SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER) {   // :73-108
+            std::fill(out, out + spec, cplx(0, 0));
+            out[0] = cplx(ic.value * grid, 0.0);
+            if (ic.kBT == 0.0) return;
+            for (const Mode& m : active) {
+                if (m.k2 == 0.0) continue;
+                const double coef = ic.k0 + ic.k2 * m.k2 + ic.k4 * m.k2 * m.k2;
+                if (coef <= 0.0)
+                    throw std::runtime_error("set order_parameter equilibrium/gaussian requires positive linear "
+                                             "chemical-potential coefficient for every active nonzero mode.");
+                const double sigma = std::sqrt(ic.kBT * grid / (2.0 * coef * cell));
+                out[m.index] = sigma * ic_gaussian(ic.seed, ic.component, m.gx, signed_ky(m.gy));
+            }
+        } else if (ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE ||
+                   ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE) {
+            // compressible :90-112, incompressible :90-115
+            const bool comp = ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE;
+            std::fill(out, out + spec, cplx(0, 0));
+            if (ic.kBT == 0.0) return;
+            const double rho0 = state_rho0();
+            if (rho0 <= 0.0)
+                throw std::runtime_error(comp ? "set momentum equilibrium/gaussian/compressible requires positive reference density."
+                                              : "set momentum equilibrium/gaussian/incompressible requires positive reference density.");
+            const double sigma = std::sqrt(ic.kBT * grid * rho0 / (2.0 * cell));
+            for (const Mode& m : active) {
+                if (m.k2 == 0.0) continue;
+                if (comp) {
+                    out[m.index] = sigma * ic_gaussian(ic.seed, ic.component, m.gx, signed_ky(m.gy));
+                } else {
+                    const double k = std::sqrt(m.k2);
+                    const double c = ic.component == 0 ? -m.ky / k : m.kx / k;
+                    out[m.index] = c * (sigma * ic_gaussian(ic.seed, 0, m.gx, signed_ky(m.gy)));
+                }
+            }
+        } else {
+            // random_vorticity :101-147; the spectrum style normalizes the same way
+            double norm = 0.0;
+            for (const Mode& m : active) {
+                cplx ux, uy;
+                ic_velocity(ic, m, ux, uy);
+                norm += (m.gx == 0 ? 1.0 : 2.0) * (std::norm(ux) + std::norm(uy));
+            }
+            if (norm == 0.0) throw std::runtime_error("set momentum random_vorticity generated zero velocity field.");
+            const double rho0 = state_rho0();
+            if (rho0 == 0.0) throw std::runtime_error("set momentum random_vorticity requires nonzero reference density.");
+            const double scale = ic.urms * grid / std::sqrt(norm);
+            std::fill(out, out + spec, cplx(0, 0));
+            for (const Mode& m : active) {
+                cplx ux, uy;
+                ic_velocity(ic, m, ux, uy);
+                out[m.index] = rho0 * scale * (ic.component == 0 ? ux : uy);
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------------
     // FluxBuffer accumulation (src/buffer_flux.cc:170-225)
     // ---------------------------------------------------------------------
     void flux_begin(std::vector<cplx>& b) {
@@ -1188,6 +1341,22 @@ int oracle_get_flux(const oracle_solver* s, double* out, size_t n) {
     std::memcpy(out, o.flux_total.data(), sizeof(double) * n);
     return SHHD_OK;
 }
+
+int oracle_apply_initial_condition(oracle_solver* s, const shhd_initial_condition* ic) {
+    Oracle& o = *s->impl;
+    try {
+        o.apply_ic(*ic);
+        o.physical_ready = false;
+        o.cached_state = nullptr;
+        o.rho0_ready = false;
+        return SHHD_OK;
+    } catch (const std::exception& e) {
+        o.error = e.what();
+        return SHHD_ERR_INVALID;
+    }
+}
+
+unsigned long long oracle_splitmix64(unsigned long long v) { return ic_splitmix64(v); }
 
 int oracle_fft_c2c(int n, const double* in, double* out, int sign) {
     try {

@@ -42,6 +42,11 @@ int  oracle_rhs_sto(oracle_solver* s, double* out);
 int  oracle_set_flux_request(oracle_solver* s, int density, int order_parameter, int momentum);
 int  oracle_get_flux(const oracle_solver* s, double* out, size_t n);
 
+/* `set` initial conditions (src/initial_condition_*.cc) applied to the
+ * oracle's state; splitmix64 as the mode hash uses it. */
+int  oracle_apply_initial_condition(oracle_solver* s, const shhd_initial_condition* ic);
+unsigned long long oracle_splitmix64(unsigned long long v);
+
 /* Transforms used by the oracle (pinned against the DFT definition). */
 int  oracle_fft_c2c(int n, const double* in, double* out, int sign);
 int  oracle_fft2_r2c(int mx, int my, const double* phys, double* spec);

@@ -44,6 +44,9 @@ def load():
     lib.oracle_rhs_sto.argtypes = [P, D]
     lib.oracle_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.oracle_get_flux.argtypes = [P, D, ctypes.c_size_t]
+    lib.oracle_apply_initial_condition.argtypes = [P, ctypes.POINTER(shhd.ShhdInitialCondition)]
+    lib.oracle_splitmix64.argtypes = [ctypes.c_ulonglong]
+    lib.oracle_splitmix64.restype = ctypes.c_ulonglong
     lib.oracle_fft_c2c.argtypes = [ctypes.c_int, D, D, ctypes.c_int]
     lib.oracle_fft2_r2c.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
     lib.oracle_fft2_c2r.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
@@ -116,10 +119,17 @@ class Oracle:
         self._check(self.lib.oracle_get_flux(self.h, _d(f), f.size))
         return out
 
+    def apply_initial_condition(self, ic):
+        self._check(self.lib.oracle_apply_initial_condition(self.h, ctypes.byref(ic)))
+
     def rhs_sto(self):
         out = np.zeros(self.cfg.state_shape(), dtype=np.complex128)
         self._check(self.lib.oracle_rhs_sto(self.h, _d(out.view(np.float64))))
         return out
+
+
+def splitmix64(v):
+    return int(load().oracle_splitmix64(int(v)))
 
 
 def fft_c2c(x, sign=-1):

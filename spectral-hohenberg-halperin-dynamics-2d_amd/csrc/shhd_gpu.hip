@@ -31,6 +31,7 @@
 #include "../../include/shhd_gpu.h"
 #include "instances.h"
 #include "measure.h"
+#include "ic.h"
 
 using namespace shhd;
 
@@ -1080,6 +1081,141 @@ struct Engine {
         return SHHD_OK;
     }
 
+    // `set <target> <style>` evaluated on the device into the resident state
+    // (src/initial_condition_*.cc); argument checks and messages follow the
+    // styles' parsers and apply() functions.
+    shhd_status apply_initial_condition(int slot, const shhd_initial_condition* icp) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (!icp) return set_err(&err, SHHD_ERR_INVALID, "null initial condition");
+        const shhd_initial_condition ic = *icp;
+        const shhd_config& c = h->cfg;
+        const int N = h->desc.N;
+        int f = -1;
+        switch (ic.kind) {
+            case SHHD_IC_EQUILIBRIUM_DENSITY:
+                if (ic.kBT < 0.0) return set_err(&err, SHHD_ERR_INVALID, "set density equilibrium/gaussian requires kBT >= 0.");
+                if (ic.value <= 0.0) return set_err(&err, SHHD_ERR_INVALID, "set density equilibrium/gaussian requires rho0 > 0.");
+                if (ic.pressure_coefficient <= 0.0)
+                    return set_err(&err, SHHD_ERR_INVALID, "set density equilibrium/gaussian requires cT > 0.");
+                f = 0;
+                break;
+            case SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER:
+                if (ic.component < 0 || ic.component >= N)
+                    return set_err(&err, SHHD_ERR_INVALID, "order-parameter index out of range");
+                if (ic.kBT < 0.0) return set_err(&err, SHHD_ERR_INVALID, "set order_parameter equilibrium/gaussian requires kBT >= 0.");
+                f = 1 + ic.component;
+                break;
+            case SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE:
+            case SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE:
+            case SHHD_IC_RANDOM_VORTICITY:
+            case SHHD_IC_SPECTRUM_K4:
+                if (ic.component != 0 && ic.component != 1)
+                    return set_err(&err, SHHD_ERR_INVALID, "momentum direction must be 0 or 1");
+                if ((ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE ||
+                     ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE) && ic.kBT < 0.0)
+                    return set_err(&err, SHHD_ERR_INVALID, std::string("set momentum ") +
+                                   (ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE
+                                        ? "equilibrium/gaussian/compressible" : "equilibrium/gaussian/incompressible") +
+                                   " requires kBT >= 0.");
+                if (ic.kind == SHHD_IC_RANDOM_VORTICITY || ic.kind == SHHD_IC_SPECTRUM_K4) {
+                    const char* nm = ic.kind == SHHD_IC_RANDOM_VORTICITY ? "random_vorticity" : "spectrum/k4";
+                    if (ic.n0 <= 0.0) return set_err(&err, SHHD_ERR_INVALID, std::string("set momentum ") + nm + " requires positive n0.");
+                    if (ic.kind == SHHD_IC_RANDOM_VORTICITY && ic.sigma <= 0.0)
+                        return set_err(&err, SHHD_ERR_INVALID, "set momentum random_vorticity requires positive sigma.");
+                    if (ic.urms <= 0.0) return set_err(&err, SHHD_ERR_INVALID, std::string("set momentum ") + nm + " requires positive urms.");
+                }
+                f = N + 1 + ic.component;
+                break;
+            default:
+                return set_err(&err, SHHD_ERR_INVALID, "unknown initial-condition kind");
+        }
+        IcDev d{};
+        d.ic = ic;
+        d.nay = c.ny_active;
+        d.hxa = c.nx_active / 2;
+        d.mx = c.nx;
+        d.my = c.ny;
+        d.lx = c.lx;
+        d.ly = c.ly;
+        d.grid = static_cast<double>(c.nx) * static_cast<double>(c.ny);
+        d.cell = c.lx * c.ly / d.grid;
+        const double two_pi = 2.0 * 3.14159265358979323846;
+        if (ic.kind == SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER && ic.kBT != 0.0) {
+            for (int gx = 0; gx < d.hxa; ++gx) {
+                for (int ky = -(d.nay / 2 - 1); ky < d.nay / 2; ++ky) {
+                    const double kx = two_pi * gx / c.lx, kyv = two_pi * ky / c.ly;
+                    const double k2 = kx * kx + kyv * kyv;
+                    if (k2 == 0.0) continue;
+                    if (ic.k0 + ic.k2 * k2 + ic.k4 * k2 * k2 <= 0.0)
+                        return set_err(&err, SHHD_ERR_INVALID,
+                                       "set order_parameter equilibrium/gaussian requires positive linear chemical-potential "
+                                       "coefficient for every active nonzero mode.");
+                }
+            }
+        }
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        const int b0 = h->is_static[0] ? 0 : h->cur;
+        double rho_hat0 = 0.0;
+        if (h->fp32) {
+            cx<double> v;
+            SHHD_HIP(&err, hipMemcpy(&v, h->k0[b0], sizeof(v), hipMemcpyDeviceToHost));
+            rho_hat0 = v.re;
+        } else {
+            cx<double> v;
+            SHHD_HIP(&err, hipMemcpy(&v, h->state[b0], sizeof(v), hipMemcpyDeviceToHost));
+            rho_hat0 = v.re;
+        }
+        d.rho0 = rho_hat0 / d.grid;
+        const size_t total = static_cast<size_t>(d.hxa) * d.nay;
+        const int nblk = static_cast<int>(std::min<size_t>(2048, (total + 255) / 256));
+        if (ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE || ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE) {
+            if (ic.kBT != 0.0 && d.rho0 <= 0.0)
+                return set_err(&err, SHHD_ERR_INVALID, std::string("set momentum ") +
+                               (ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE
+                                    ? "equilibrium/gaussian/compressible" : "equilibrium/gaussian/incompressible") +
+                               " requires positive reference density.");
+        }
+        if (ic.kind == SHHD_IC_RANDOM_VORTICITY || ic.kind == SHHD_IC_SPECTRUM_K4) {
+            double* partials = nullptr;
+            SHHD_HIP(&err, hipMalloc(&partials, nblk * sizeof(double)));
+            hipLaunchKernelGGL(k_ic_norm, dim3(nblk), dim3(256), 0, stream, d, partials);
+            std::vector<double> ps(static_cast<size_t>(nblk));
+            const hipError_t e1 = hipGetLastError();
+            const hipError_t e2 = hipMemcpyAsync(ps.data(), partials, nblk * sizeof(double), hipMemcpyDeviceToHost, stream);
+            const hipError_t e3 = hipStreamSynchronize(stream);
+            (void)hipFree(partials);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+                return set_err(&err, SHHD_ERR_HIP, "initial-condition normalization failed");
+            double norm = 0.0;
+            for (double p : ps) norm += p;
+            if (norm == 0.0) return set_err(&err, SHHD_ERR_INVALID, "set momentum random_vorticity generated zero velocity field.");
+            if (d.rho0 == 0.0) return set_err(&err, SHHD_ERR_INVALID, "set momentum random_vorticity requires nonzero reference density.");
+            d.scale = ic.urms * d.grid / std::sqrt(norm);
+        }
+        const int b = h->is_static[f] ? 0 : h->cur;
+        const size_t off = static_cast<size_t>(f) * h->field_elems;
+        if (h->fp32)
+            hipLaunchKernelGGL(k_initial_condition<float>, dim3(nblk), dim3(256), 0, stream, d,
+                               static_cast<cx<float>*>(h->state[b]) + off);
+        else
+            hipLaunchKernelGGL(k_initial_condition<double>, dim3(nblk), dim3(256), 0, stream, d,
+                               static_cast<cx<double>*>(h->state[b]) + off);
+        SHHD_HIP(&err, hipGetLastError());
+        double k0re = 0.0;
+        if (ic.kind == SHHD_IC_EQUILIBRIUM_DENSITY || ic.kind == SHHD_IC_EQUILIBRIUM_ORDER_PARAMETER) k0re = ic.value * d.grid;
+        if (h->fp32) {
+            const cx<double> v = mk(k0re, 0.0);
+            SHHD_HIP(&err, hipMemcpyAsync(h->k0[b] + f, &v, sizeof(v), hipMemcpyHostToDevice, stream));
+        }
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        if (f == 0) {
+            h->desc.rho0 = k0re / d.grid;   // as upload_state derives it
+            dirty = true;
+        }
+        return SHHD_OK;
+    }
+
     // FluidSolver::flux_buffer (src/fluid_solver.h:105-107) in the reference
     // layout: slots of complex[(Mx/2+1) * My], inactive modes 0.
     shhd_status get_flux(int slot, double* out, size_t n) {
@@ -1585,6 +1721,13 @@ shhd_status shhd_gpu_get_structure_factor(shhd_gpu* h, double* out, size_t n, lo
     return h->eng.get_structure_factor(h->slot, out, n, nsamples);
 }
 
+shhd_status shhd_gpu_apply_initial_condition(shhd_gpu* h, const shhd_initial_condition* ic) {
+    if (!h) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    const shhd_status st = h->eng.apply_initial_condition(h->slot, ic);
+    if (st == SHHD_OK) h->state_set = true;   // the state starts at zero, as a new reference State does
+    return st;
+}
+
 void shhd_gpu_destroy(shhd_gpu* h) { delete h; }
 
 const char* shhd_gpu_last_error(const shhd_gpu* h) { return h ? h->eng.err.c_str() : g_create_error.c_str(); }
@@ -1699,6 +1842,11 @@ shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned fi
 shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples) {
     if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
     return b->eng.get_structure_factor(slot, out, n, nsamples);
+}
+
+shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
+    return b->eng.apply_initial_condition(slot, ic);
 }
 
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes) {

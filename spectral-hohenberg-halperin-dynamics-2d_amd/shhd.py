@@ -46,6 +46,25 @@ class ShhdProbe(ctypes.Structure):
 
 PROBE_STATE, PROBE_FLUX = 0, 1
 
+(IC_EQUILIBRIUM_DENSITY, IC_EQUILIBRIUM_ORDER_PARAMETER, IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE,
+ IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE, IC_RANDOM_VORTICITY, IC_SPECTRUM_K4) = range(6)
+
+
+class ShhdInitialCondition(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("component", ctypes.c_int), ("seed", ctypes.c_int),
+                ("kBT", ctypes.c_double), ("value", ctypes.c_double), ("pressure_coefficient", ctypes.c_double),
+                ("k0", ctypes.c_double), ("k2", ctypes.c_double), ("k4", ctypes.c_double),
+                ("n0", ctypes.c_double), ("sigma", ctypes.c_double), ("urms", ctypes.c_double)]
+
+
+def initial_condition(kind, component=0, seed=12345, kBT=0.0, value=0.0, pressure_coefficient=0.0, k0=0.0,
+                      k2=0.0, k4=0.0, n0=0.0, sigma=0.0, urms=0.0) -> ShhdInitialCondition:
+    """Lowered `set` command (include/shhd_gpu.h shhd_initial_condition); for
+    density equilibrium/gaussian pass pressure_coefficient = cT^2."""
+    return ShhdInitialCondition(int(kind), int(component), int(seed), float(kBT), float(value),
+                                float(pressure_coefficient), float(k0), float(k2), float(k4), float(n0),
+                                float(sigma), float(urms))
+
 
 class ShhdGradientForce(ctypes.Structure):
     _fields_ = [("component", ctypes.c_int), ("direction", ctypes.c_int), ("amplitude", ctypes.c_double)]
@@ -270,6 +289,9 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
     for pre, first in (("shhd_gpu_", [P]), ("shhd_batch_", [P, ctypes.c_int])):
+        f = getattr(lib, pre + "apply_initial_condition")
+        f.argtypes = first + [ctypes.POINTER(ShhdInitialCondition)]
+        f.restype = ctypes.c_int
         f = getattr(lib, pre + "set_probes")
         f.argtypes = first + [ctypes.POINTER(ShhdProbe), ctypes.c_int, ctypes.c_int]
         f.restype = ctypes.c_int
@@ -377,6 +399,10 @@ class GpuFluidSolver(_Measures):
     def _call(self, name, key, *args):
         return getattr(self.lib, "shhd_gpu_" + name)(self.h, *args)
 
+    def apply_initial_condition(self, ic: ShhdInitialCondition):
+        """`set <target> <style>` on the device (shhd_gpu_apply_initial_condition)."""
+        self._check(self.lib.shhd_gpu_apply_initial_condition(self.h, ctypes.byref(ic)))
+
     def set_probes(self, probes, capacity):
         """probes: list of (source, index, gx, gy); source PROBE_STATE or PROBE_FLUX."""
         self._set_probes(0, probes, capacity)
@@ -479,6 +505,9 @@ class Batch(_Measures):
 
     def _call(self, name, key, *args):
         return getattr(self.lib, "shhd_batch_" + name)(self.h, int(key), *args)
+
+    def apply_initial_condition(self, slot, ic: ShhdInitialCondition):
+        self._check(self.lib.shhd_batch_apply_initial_condition(self.h, int(slot), ctypes.byref(ic)))
 
     def set_probes(self, slot, probes, capacity):
         self._set_probes(slot, probes, capacity)
