@@ -232,6 +232,23 @@ typedef struct shhd_initial_condition {
  * reference's messages. */
 shhd_status shhd_gpu_apply_initial_condition(shhd_gpu* h, const shhd_initial_condition* ic);
 
+/* ---- snapshots (SnapshotMeasure, src/measure_snapshot.cc) -------------------- */
+
+/* Every field in physical space on the padded grid, as the reference's
+ * PhysicalStateBuffer after FourierTransform2D::backward_many
+ * (src/measure_workspace.h:27-35, src/buffer_physical_state.h): out[f][gy][gx],
+ * n = (N+3) * Mx * My doubles, computed on the device (inverse column pass of
+ * every field, then the row half). */
+shhd_status shhd_gpu_get_physical(shhd_gpu* h, double* out, size_t n);
+
+enum { SHHD_SNAPSHOT_PHYSICAL = 0, SHHD_SNAPSHOT_SPECTRAL = 1 };
+/* Write `data` as the reference's snapshot text (write_physical_snapshot /
+ * write_spectral_snapshot, src/measure_snapshot.cc:54-138): physical data as
+ * from shhd_gpu_get_physical, spectral data as from shhd_gpu_get_state;
+ * nx, ny = the padded grid Mx, My.  Host-only: touches no device. */
+shhd_status shhd_write_snapshot(const char* path, int space, const double* data, int nx, int ny,
+                                int num_order_parameters, int step, double time);
+
 /* Destroy the handle and free its device memory. */
 void shhd_gpu_destroy(shhd_gpu* h);
 
@@ -288,6 +305,8 @@ shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned fi
 shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples);
 /* initial condition of one instance, as shhd_gpu_apply_initial_condition */
 shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic);
+/* physical fields of one instance, as shhd_gpu_get_physical */
+shhd_status shhd_batch_get_physical(shhd_batch* b, int slot, double* out, size_t n);
 /* free an instance's device memory and slot */
 shhd_status shhd_batch_retire(shhd_batch* b, int slot);
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes);

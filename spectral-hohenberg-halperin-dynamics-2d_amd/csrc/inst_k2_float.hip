@@ -6,4 +6,6 @@ namespace shhd {
     template __global__ void k_rows<float, nt, v, mn>(const InstDev<float>*, BlockMap);
 #define SHHD_INST(nt, v) SHHD_INST_N(nt, v, 0) SHHD_INST_N(nt, v, 1) SHHD_INST_N(nt, v, 2) SHHD_INST_N(nt, v, 4) SHHD_INST_N(nt, v, 8)
 SHHD_FOR_NT_V(SHHD_INST)
+#define SHHD_INST_P(nt, v) template __global__ void k_rows_physical<float, nt, v>(const InstDev<float>*, float*, int);
+SHHD_FOR_NT_V(SHHD_INST_P)
 }  // namespace shhd

@@ -17,8 +17,11 @@ namespace shhd {
 #define SHHD_DECL_K2(T, nt, v, mn) \
     extern template __global__ void k_rows<T, nt, v, mn>(const InstDev<T>*, BlockMap);
 
+#define SHHD_DECL_KP(T, nt, v) \
+    extern template __global__ void k_rows_physical<T, nt, v>(const InstDev<T>*, T*, int);
+
 #define SHHD_DECL_ALL(T, nt, v)                                                              \
-    SHHD_DECL_K1(T, nt, v) SHHD_DECL_K3(T, nt, v, 4) SHHD_DECL_K3(T, nt, v, v)                \
+    SHHD_DECL_K1(T, nt, v) SHHD_DECL_K3(T, nt, v, 4) SHHD_DECL_K3(T, nt, v, v) SHHD_DECL_KP(T, nt, v) \
     SHHD_DECL_K2(T, nt, v, 0) SHHD_DECL_K2(T, nt, v, 1) SHHD_DECL_K2(T, nt, v, 2)            \
     SHHD_DECL_K2(T, nt, v, 4) SHHD_DECL_K2(T, nt, v, 8)
 #define SHHD_DECL_F64(nt, v) SHHD_DECL_ALL(double, nt, v)

@@ -248,6 +248,69 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 }
 
 // ---------------------------------------------------------------------------
+// Physical fields for a snapshot (PhysicalStateBuffer of the reference,
+// src/buffer_physical_state.h, filled by FourierTransform2D::backward_many):
+// the row half of the inverse transform of every field, after K1 has done the
+// column half into T.  Same pairing of two real fields per complex FFT as
+// K2; rows are written to out[f][y][0..Mx).
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int PPT>
+__global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restrict__ insts, T* __restrict__ out,
+                                                      int nfields) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const InstDev<T>& I = insts[0];
+    const Desc& d = I.d;
+    const cx<T>* __restrict__ tbuf = I.tbuf;
+    const int R = I.R;
+    const int L = d.mx;
+    const int my = d.my;
+    const int hxa = d.hxa;
+    const int y0 = blockIdx.x * R;
+    const int npts = R * L;
+    const size_t plane = static_cast<size_t>(my) * hxa;
+    for (int s = 0; s < nfields; s += 2) {
+        const bool has_b = s + 1 < nfields;
+        const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
+        const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
+        for (int idx = threadIdx.x; idx < npts; idx += NT) {
+            const int r = idx / L;
+            const int k = idx - r * L;
+            const int y = y0 + r;
+            cx<T> z = czero<T>();
+            if (y < my) {
+                const size_t row = static_cast<size_t>(y) * hxa;
+                if (k < hxa) {
+                    cx<T> a = ta[row + k];
+                    cx<T> b = has_b ? tb[row + k] : czero<T>();
+                    if (k == 0) { a.im = T(0); b.im = T(0); }
+                    z = mk(a.re - b.im, a.im + b.re);
+                } else if (k > L - hxa) {
+                    const int kk = L - k;
+                    const cx<T> a = ta[row + kk];
+                    const cx<T> b = has_b ? tb[row + kk] : czero<T>();
+                    z = mk(a.re + b.im, b.re - a.im);
+                }
+            }
+            lds[lp(idx)] = z;
+        }
+        __syncthreads();
+        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, I.twx, +1);
+        for (int idx = threadIdx.x; idx < npts; idx += NT) {
+            const int r = idx / L;
+            const int x = idx - r * L;
+            const int y = y0 + r;
+            if (y < my) {
+                const cx<T> z = lds[lp(idx)];
+                out[(static_cast<size_t>(s) * my + y) * L + x] = z.re;
+                if (has_b) out[(static_cast<size_t>(s + 1) * my + y) * L + x] = z.im;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // noise: one complex N(0,1) pair per Philox call
 // ---------------------------------------------------------------------------
 template <typename T>

@@ -170,6 +170,17 @@ K2Fn<T> pick_k2(int NT, int V, int maxn) {
     return nullptr;
 }
 
+template <typename T>
+using KPFn = void (*)(const InstDev<T>*, T*, int);
+
+template <typename T>
+KPFn<T> pick_kp(int NT, int V) {
+#define SHHD_KP(nt, v) if (NT == nt && V == v) return &k_rows_physical<T, nt, v>;
+    SHHD_FOR_NT_V(SHHD_KP)
+#undef SHHD_KP
+    return nullptr;
+}
+
 __global__ void k_clear_flags(int* flags, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flags[i] = 0;
@@ -1216,6 +1227,87 @@ struct Engine {
         return SHHD_OK;
     }
 
+    // All fields in physical space on the padded grid, as the reference's
+    // PhysicalStateBuffer after FourierTransform2D::backward_many
+    // (src/measure_workspace.h:27-35): out[f][gy][gx], n = nfields*Mx*My.
+    // K1 over every field into scratch planes, then the row half.
+    template <typename T>
+    shhd_status physical_fields(Instance& h, double* out) {
+        const int nf = h.desc.nfields;
+        const shhd_config& c = h.cfg;
+        const size_t nphys = static_cast<size_t>(c.nx) * c.ny;
+        InstDev<T> I{};
+        I.d = h.desc;
+        I.d.n_in = nf;
+        for (int f = 0; f < nf; ++f) I.d.in_field[f] = f;
+        I.twx = static_cast<const cx<T>*>(h.twx);
+        I.twy = static_cast<const cx<T>*>(h.twy);
+        I.C = h.cols.C;
+        I.R = h.rows.R;
+        I.ntiles = h.cols.ntiles;
+        const StagePtrs<T> P = stage_ptrs<T>(h, h.cur, h.cur, h.cur);
+        const int nblk1 = h.cols.ntiles * nf;
+        std::vector<int> map(static_cast<size_t>(nblk1) + 1, 0);   // every block -> instance 0; first[0] = 0
+        void* d_t = nullptr;
+        void* d_out = nullptr;
+        void* d_tab = nullptr;
+        shhd_status st = SHHD_OK;
+        auto fail = [&](hipError_t e, const char* what) {
+            st = set_err(&err, SHHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+        };
+        hipError_t e = hipMalloc(&d_t, static_cast<size_t>(nf) * h.plane_elems * sizeof(cx<T>));
+        if (e != hipSuccess) fail(e, "hipMalloc");
+        if (st == SHHD_OK && (e = hipMalloc(&d_out, nf * nphys * sizeof(T))) != hipSuccess) fail(e, "hipMalloc");
+        const size_t tab_bytes = sizeof(InstDev<T>) + sizeof(StagePtrs<T>) + map.size() * sizeof(int);
+        if (st == SHHD_OK && (e = hipMalloc(&d_tab, tab_bytes)) != hipSuccess) fail(e, "hipMalloc");
+        if (st == SHHD_OK) {
+            I.tbuf = static_cast<cx<T>*>(d_t);
+            unsigned char* base = static_cast<unsigned char*>(d_tab);
+            InstDev<T>* dI = reinterpret_cast<InstDev<T>*>(base);
+            StagePtrs<T>* dP = reinterpret_cast<StagePtrs<T>*>(base + sizeof(InstDev<T>));
+            int* dmap = reinterpret_cast<int*>(base + sizeof(InstDev<T>) + sizeof(StagePtrs<T>));
+            BlockMap bm{dmap, dmap + nblk1};
+            if ((e = hipMemcpy(dI, &I, sizeof(I), hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(dP, &P, sizeof(P), hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(dmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) {
+                fail(e, "hipMemcpy");
+            } else {
+                K1Fn<T> k1 = pick_k1<T>(h.cols.NT, h.cols.V);
+                KPFn<T> kp = pick_kp<T>(h.rows.NT, h.rows.V);
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k1), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          static_cast<int>(h.cols.lds));
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          static_cast<int>(h.rows.lds));
+                hipLaunchKernelGGL(k1, dim3(nblk1), dim3(h.cols.NT), h.cols.lds, stream, dI, dP, bm);
+                hipLaunchKernelGGL(kp, dim3(h.rows.nblocks), dim3(h.rows.NT), h.rows.lds, stream, dI,
+                                   static_cast<T*>(d_out), nf);
+                if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(stream)) != hipSuccess) {
+                    fail(e, "physical transform");
+                } else {
+                    std::vector<T> host(nf * nphys);
+                    if ((e = hipMemcpy(host.data(), d_out, host.size() * sizeof(T), hipMemcpyDeviceToHost)) != hipSuccess)
+                        fail(e, "hipMemcpy");
+                    else
+                        for (size_t i = 0; i < host.size(); ++i) out[i] = static_cast<double>(host[i]);
+                }
+            }
+        }
+        if (d_t) (void)hipFree(d_t);
+        if (d_out) (void)hipFree(d_out);
+        if (d_tab) (void)hipFree(d_tab);
+        return st;
+    }
+
+    shhd_status get_physical(int slot, double* out, size_t n) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        const shhd_config& c = h->cfg;
+        if (n != static_cast<size_t>(h->desc.nfields) * c.nx * c.ny)
+            return set_err(&err, SHHD_ERR_INVALID, "physical size mismatch");
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        return h->fp32 ? physical_fields<float>(*h, out) : physical_fields<double>(*h, out);
+    }
+
     // FluidSolver::flux_buffer (src/fluid_solver.h:105-107) in the reference
     // layout: slots of complex[(Mx/2+1) * My], inactive modes 0.
     shhd_status get_flux(int slot, double* out, size_t n) {
@@ -1728,6 +1820,72 @@ shhd_status shhd_gpu_apply_initial_condition(shhd_gpu* h, const shhd_initial_con
     return st;
 }
 
+// SnapshotMeasure::write_physical_snapshot / write_spectral_snapshot
+// (src/measure_snapshot.cc:54-138): the same header lines and columns,
+// numbers as std::scientific with 16 digits (printf "%.16e").
+shhd_status shhd_write_snapshot(const char* path, int space, const double* data, int nx, int ny,
+                                int num_order_parameters, int step, double time) {
+    if (!path || !data || nx <= 0 || ny <= 0 || num_order_parameters < 0 ||
+        (space != SHHD_SNAPSHOT_PHYSICAL && space != SHHD_SNAPSHOT_SPECTRAL))
+        return set_err(nullptr, SHHD_ERR_INVALID, "invalid snapshot arguments");
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return set_err(nullptr, SHHD_ERR_RUNTIME, std::string("SnapshotMeasure: cannot open file: ") + path);
+    const int N = num_order_parameters;
+    const bool phys = space == SHHD_SNAPSHOT_PHYSICAL;
+    const int nxs = phys ? nx : nx / 2 + 1;
+    const size_t field = static_cast<size_t>(nxs) * ny;
+    std::string line;
+    char num[40];
+    std::fprintf(fp, "# snapshot space %s\n", phys ? "physical" : "spectral");
+    std::fprintf(fp, "# step %d time %.16e\n", step, time);
+    std::fprintf(fp, phys ? "# nx %d ny %d order_parameters %d\n" : "# nkx %d nky %d order_parameters %d\n", nxs, ny, N);
+    line = phys ? "# x y rho" : "# kx_index ky_index rho_real rho_imag";
+    for (int a = 0; a < N; ++a) {
+        line += phys ? " psi_com" + std::to_string(a)
+                     : " psi_com" + std::to_string(a) + "_real psi_com" + std::to_string(a) + "_imag";
+    }
+    line += phys ? " jx jy\n" : " jx_real jx_imag jy_real jy_imag\n";
+    std::fputs(line.c_str(), fp);
+    const int nf = N + 3;
+    std::string buf;
+    buf.reserve(1 << 20);
+    for (int gy = 0; gy < ny; ++gy) {
+        for (int gx = 0; gx < nxs; ++gx) {
+            const size_t i = static_cast<size_t>(gy) * nxs + gx;
+            buf += std::to_string(gx);
+            buf += ' ';
+            buf += std::to_string(gy);
+            for (int f = 0; f < nf; ++f) {
+                if (phys) {
+                    std::snprintf(num, sizeof(num), " %.16e", data[static_cast<size_t>(f) * field + i]);
+                    buf += num;
+                } else {
+                    const size_t o = 2 * (static_cast<size_t>(f) * field + i);
+                    std::snprintf(num, sizeof(num), " %.16e", data[o]);
+                    buf += num;
+                    std::snprintf(num, sizeof(num), " %.16e", data[o + 1]);
+                    buf += num;
+                }
+            }
+            buf += '\n';
+            if (buf.size() > (1u << 20) - 1024) {
+                std::fwrite(buf.data(), 1, buf.size(), fp);
+                buf.clear();
+            }
+        }
+    }
+    std::fwrite(buf.data(), 1, buf.size(), fp);
+    const bool ok = std::ferror(fp) == 0;
+    if (std::fclose(fp) != 0 || !ok)
+        return set_err(nullptr, SHHD_ERR_RUNTIME, std::string("SnapshotMeasure: cannot write file: ") + path);
+    return SHHD_OK;
+}
+
+shhd_status shhd_gpu_get_physical(shhd_gpu* h, double* out, size_t n) {
+    if (!h || !out) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return h->eng.get_physical(h->slot, out, n);
+}
+
 void shhd_gpu_destroy(shhd_gpu* h) { delete h; }
 
 const char* shhd_gpu_last_error(const shhd_gpu* h) { return h ? h->eng.err.c_str() : g_create_error.c_str(); }
@@ -1847,6 +2005,11 @@ shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out
 shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic) {
     if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
     return b->eng.apply_initial_condition(slot, ic);
+}
+
+shhd_status shhd_batch_get_physical(shhd_batch* b, int slot, double* out, size_t n) {
+    if (!b || !out) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return b->eng.get_physical(slot, out, n);
 }
 
 shhd_status shhd_batch_usage(const shhd_batch* b, int* active, size_t* used_bytes) {

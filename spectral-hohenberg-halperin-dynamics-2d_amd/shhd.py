@@ -288,7 +288,13 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_write_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.shhd_write_snapshot.restype = ctypes.c_int
     for pre, first in (("shhd_gpu_", [P]), ("shhd_batch_", [P, ctypes.c_int])):
+        f = getattr(lib, pre + "get_physical")
+        f.argtypes = first + [ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+        f.restype = ctypes.c_int
         f = getattr(lib, pre + "apply_initial_condition")
         f.argtypes = first + [ctypes.POINTER(ShhdInitialCondition)]
         f.restype = ctypes.c_int
@@ -399,6 +405,12 @@ class GpuFluidSolver(_Measures):
     def _call(self, name, key, *args):
         return getattr(self.lib, "shhd_gpu_" + name)(self.h, *args)
 
+    def get_physical(self) -> np.ndarray:
+        """Every field on the padded physical grid, shape (N+3, My, Mx)."""
+        out = np.zeros((self.cfg.num_fields, self.cfg.ny, self.cfg.nx))
+        self._check(self.lib.shhd_gpu_get_physical(self.h, _dptr(out.reshape(-1)), out.size))
+        return out
+
     def apply_initial_condition(self, ic: ShhdInitialCondition):
         """`set <target> <style>` on the device (shhd_gpu_apply_initial_condition)."""
         self._check(self.lib.shhd_gpu_apply_initial_condition(self.h, ctypes.byref(ic)))
@@ -506,6 +518,12 @@ class Batch(_Measures):
     def _call(self, name, key, *args):
         return getattr(self.lib, "shhd_batch_" + name)(self.h, int(key), *args)
 
+    def get_physical(self, slot) -> np.ndarray:
+        cfg = self.cfgs[slot]
+        out = np.zeros((cfg.num_fields, cfg.ny, cfg.nx))
+        self._check(self.lib.shhd_batch_get_physical(self.h, int(slot), _dptr(out.reshape(-1)), out.size))
+        return out
+
     def apply_initial_condition(self, slot, ic: ShhdInitialCondition):
         self._check(self.lib.shhd_batch_apply_initial_condition(self.h, int(slot), ctypes.byref(ic)))
 
@@ -589,3 +607,57 @@ class Batch(_Measures):
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# snapshots (SnapshotMeasure, src/measure_snapshot.cc)
+# ---------------------------------------------------------------------------
+SNAPSHOT_PHYSICAL, SNAPSHOT_SPECTRAL = 0, 1
+
+
+def snapshot_filename(file: str, step: int, total_run_steps: int) -> str:
+    """`<file>_step<zero-padded step>.snapshot`, width = digits of the run's
+    total steps + 1 (src/measure_snapshot.cc:35-52)."""
+    digits, m = 1, int(total_run_steps)
+    while m >= 10:
+        m //= 10
+        digits += 1
+    return f"{file}_step{int(step):0{digits + 1}d}.snapshot"
+
+
+def write_snapshot(path: str, space: int, data: np.ndarray, cfg: SolverConfig, step: int, time: float):
+    """Reference snapshot text (shhd_write_snapshot): physical data of shape
+    (N+3, My, Mx) or a spectral state of shape (N+3, My, Mx/2+1)."""
+    lib = load_library()
+    if space == SNAPSHOT_PHYSICAL:
+        arr = np.ascontiguousarray(data, dtype=np.float64)
+        assert arr.shape == (cfg.num_fields, cfg.ny, cfg.nx)
+        flat = arr.reshape(-1)
+    else:
+        arr = np.ascontiguousarray(data, dtype=np.complex128)
+        assert arr.shape == cfg.state_shape()
+        flat = arr.view(np.float64).reshape(-1)
+    st = lib.shhd_write_snapshot(path.encode(), int(space), _dptr(flat), cfg.nx, cfg.ny, cfg.num_order_parameters,
+                                 int(step), float(time))
+    if st != 0:
+        raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())
+
+
+def read_snapshot(path: str):
+    """Parse a snapshot file: returns (space, step, time, data) with data shaped
+    as write_snapshot takes it, so a spectral snapshot is a state again."""
+    with open(path) as fh:
+        head = [next(fh) for _ in range(4)]
+    space = head[0].split()[-1]
+    t = head[1].split()
+    step, time_ = int(t[2]), float(t[4])
+    d = head[2].split()
+    nxs, ny, n = int(d[2]), int(d[4]), int(d[6])
+    table = np.loadtxt(path, comments="#", ndmin=2)
+    nf = n + 3
+    if space == "physical":
+        data = table[:, 2:2 + nf].T.reshape(nf, ny, nxs)
+        return SNAPSHOT_PHYSICAL, step, time_, data
+    vals = table[:, 2:2 + 2 * nf]
+    data = (vals[:, 0::2] + 1j * vals[:, 1::2]).T.reshape(nf, ny, nxs)
+    return SNAPSHOT_SPECTRAL, step, time_, data

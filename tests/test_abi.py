@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "shhd_gpu.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(shhd_(?:gpu|batch)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(shhd_\w+)\s*\(", text)))
 
 
 def test_header_declares_reference_boundary():
