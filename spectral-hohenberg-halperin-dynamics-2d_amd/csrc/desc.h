@@ -38,11 +38,25 @@ struct OutItem {
     int alpha;
 };
 
+// What one forward-transformed product v_hat adds at mode k, written as
+// polynomials in (kx, ky) so the forward-column kernel evaluates every kind of
+// product with the same few multiply-adds:
+//   acc_r += (a_r0 + a_r1 kx^2 + a_r2 ky^2 + a_r3 kx ky) v - i (b_r0 kx + b_r1 ky) v
+//   flux_t += (f_t0 - i (f_t1 kx + f_t2 ky)) v
+// (acc 0/1: the psi or jx / jy right-hand side; flux targets 0..2: psi flux
+// x, y or Pi xx, xy, yy).  Derived from the reference's terms by build_desc.
+struct ContribCoef {
+    double a[2][4];
+    double b[2][2];
+    double f[3][3];
+};
+
 struct Group {
     int kind;        // GROUP_PSI or GROUP_MOM
     int alpha;       // order parameter of a PSI group
     int ncontrib;    // forward-transformed products feeding this group
     int contrib[kMaxContrib];   // indices into the product list (U buffer)
+    ContribCoef coef[kMaxContrib];
 };
 
 struct Source {      // constant spectral source (sine force)
@@ -57,6 +71,7 @@ struct Desc {
     // ky = c - Nay (c >= Nay/2); slot Nay/2 (ky = -Nay/2) is always zero.
     int nax, nay, mx, my, hxa;
     double lx, ly;
+    double dkx, dky;            // 2 pi / Lx, 2 pi / Ly
     int N, nfields, mode;
 
     // inverse column pass: state field of each inverse-transformed slot

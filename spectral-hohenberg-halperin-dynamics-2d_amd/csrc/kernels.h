@@ -66,7 +66,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         lds[lp(idx)] = v;
     }
     __syncthreads();
-    lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, +1);
+    lds_fft_any<T, NT, VPT>(lds, L, C, d.plan_y, twy, +1);
     cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
     for (int idx = threadIdx.x; idx < ncol; idx += NT) {
         const int y = idx / C;
@@ -84,16 +84,24 @@ struct RowFields {
     T rho[PPT], jx[PPT], jy[PPT], vx[PPT], vy[PPT];
     T psi[MAXN > 0 ? MAXN : 1][PPT];
 
-    // i must be a compile-time constant after unrolling
-    __device__ __forceinline__ void store(int role, int i, T v) {
-        if (role == ROLE_RHO) rho[i] = v;
-        else if (role == ROLE_JX) jx[i] = v;
-        else if (role == ROLE_JY) jy[i] = v;
-        else {
+    __device__ __forceinline__ void clear() {
 #pragma unroll
-            for (int a = 0; a < MAXN; ++a)
-                if (role == ROLE_PSI0 + a) psi[a][i] = v;
+        for (int i = 0; i < PPT; ++i) {
+            rho[i] = jx[i] = jy[i] = vx[i] = vy[i] = T(0);
+#pragma unroll
+            for (int a = 0; a < (MAXN > 0 ? MAXN : 1); ++a) psi[a][i] = T(0);
         }
+    }
+    // i must be a compile-time constant after unrolling.  Every array is
+    // rewritten through a select, never through a branch: branches on the
+    // runtime role let the compiler merge the stores into one store through a
+    // selected address, which puts the whole struct in scratch memory.
+    __device__ __forceinline__ void store(int role, int i, T v) {
+        rho[i] = role == ROLE_RHO ? v : rho[i];
+        jx[i] = role == ROLE_JX ? v : jx[i];
+        jy[i] = role == ROLE_JY ? v : jy[i];
+#pragma unroll
+        for (int a = 0; a < MAXN; ++a) psi[a][i] = role == ROLE_PSI0 + a ? v : psi[a][i];
     }
     __device__ __forceinline__ T psi_of(int alpha, int i) const {
         T v = T(0);
@@ -127,6 +135,17 @@ struct RowFields {
     }
 };
 
+// Row transforms of K2: the fixed-length transforms in FP32; in FP64 the
+// runtime plan, because the fixed-length code costs the FP64 row kernel the
+// few registers that separate 3 from 2 resident waves per SIMD (measured
+// slower at 1536^2).
+template <typename T, int NT, int PPT>
+__device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
+                                        const cx<T>* __restrict__ tw, int sign) {
+    if constexpr (sizeof(T) == 4) lds_fft_any<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
+    else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
+}
+
 template <typename T, int NT, int PPT, int MAXN>
 __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -149,6 +168,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
     const int tid = threadIdx.x;
 
     RowFields<T, PPT, MAXN> F;
+    F.clear();
 
     // ---- inverse x transforms, two fields per complex FFT -----------------
     for (int s = 0; s < d.n_in; s += 2) {
@@ -177,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             lds[lp(idx)] = z;
         }
         __syncthreads();
-        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, +1);
+        row_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, +1);
         const int ra = d.in_role[s];
         const int rb = has_b ? d.in_role[s + 1] : -1;
 #pragma unroll
@@ -227,7 +247,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             }
         }
         __syncthreads();
-        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, -1);
+        row_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, -1);
         cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
         cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
         const int nk = R * hxa;
@@ -295,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
             lds[lp(idx)] = z;
         }
         __syncthreads();
-        lds_fft<T, NT, PPT>(lds, L, R, d.plan_x, I.twx, +1);
+        lds_fft_any<T, NT, PPT>(lds, L, R, d.plan_x, I.twx, +1);
         for (int idx = threadIdx.x; idx < npts; idx += NT) {
             const int r = idx / L;
             const int x = idx - r * L;
@@ -428,9 +448,11 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     for (int i = 0; i < ACC; ++i) { acc0[i] = czero<T>(); acc1[i] = czero<T>(); }
 
     // ---- forward y transforms of the products feeding this group ------------
+    const bool psi_group = g.kind == GROUP_PSI;
+    const bool recording = psi_group ? fx_slot != nullptr : pi_slot != nullptr;
     for (int ci = 0; ci < g.ncontrib; ++ci) {
         const int u = g.contrib[ci];
-        const int kind = d.out[u].kind;
+        const ContribCoef& cf = g.coef[ci];
         const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
         for (int idx = tid; idx < ncol; idx += NT) {
             const int y = idx / C;
@@ -439,7 +461,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             lds[lp(c * L + y)] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
         }
         __syncthreads();
-        lds_fft<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
+        lds_fft_any<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
 #pragma unroll
         for (int i = 0; i < ACC; ++i) {
             const int m = tid + NT * i;
@@ -451,71 +473,29 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             const int ky_i = cidx < half ? cidx : cidx - nay;
             const int gy = ky_i >= 0 ? ky_i : ky_i + L;
             const cx<T> v = lds[lp(c * L + gy)];
-            const T kx = T(two_pi * gx / d.lx);
-            const T ky = T(two_pi * ky_i / d.ly);
-            const size_t at = static_cast<size_t>(gx) * nay + cidx;
-            switch (kind) {
-                case OUT_MU: {   // src/fcalculator.cc:244-256
-                    const T mob = T(d.mobility[g.alpha]);
-                    const cx<T> fx = mul_mi((mob * kx) * v);
-                    const cx<T> fy = mul_mi((mob * ky) * v);
-                    acc0[i] = acc0[i] + mul_mi(kx * fx + ky * fy);
-                    if (fx_slot) {
-                        fx_slot[at] = fx_slot[at] + fw * fx;
-                        fy_slot[at] = fy_slot[at] + fw * fy;
-                    }
-                    break;
+            const T kx = T(d.dkx) * T(gx);
+            const T ky = T(d.dky) * T(ky_i);
+            const T kxx = kx * kx, kyy = ky * ky, kxy = kx * ky;
+            const cx<T> mv = mul_mi(v);   // -i v
+            {
+                const T a = T(cf.a[0][0]) + T(cf.a[0][1]) * kxx + T(cf.a[0][2]) * kyy + T(cf.a[0][3]) * kxy;
+                const T b = T(cf.b[0][0]) * kx + T(cf.b[0][1]) * ky;
+                acc0[i] = acc0[i] + a * v + b * mv;
+            }
+            if (!psi_group) {
+                const T a = T(cf.a[1][0]) + T(cf.a[1][1]) * kxx + T(cf.a[1][2]) * kyy + T(cf.a[1][3]) * kxy;
+                const T b = T(cf.b[1][0]) * kx + T(cf.b[1][1]) * ky;
+                acc1[i] = acc1[i] + a * v + b * mv;
+            }
+            if (recording) {
+                const size_t at = static_cast<size_t>(gx) * nay + cidx;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    cx<T>* dst = psi_group ? (t == 0 ? fx_slot : (t == 1 ? fy_slot : nullptr)) : pi_slot + t * fe;
+                    if (dst == nullptr) continue;
+                    const cx<T> fv = T(cf.f[t][0]) * v + (T(cf.f[t][1]) * kx + T(cf.f[t][2]) * ky) * mv;
+                    dst[at] = dst[at] + fw * fv;
                 }
-                case OUT_PSIVX:   // src/fcalculator.cc:258-272
-                    acc0[i] = acc0[i] + mul_mi(kx * v);
-                    if (fx_slot) fx_slot[at] = fx_slot[at] + fw * v;
-                    break;
-                case OUT_PSIVY:
-                    acc0[i] = acc0[i] + mul_mi(ky * v);
-                    if (fy_slot) fy_slot[at] = fy_slot[at] + fw * v;
-                    break;
-                case OUT_GRADF: acc0[i] = acc0[i] + v; break;   // forces record no flux
-                case OUT_VX: {   // src/fcalculator.cc:378-397 with vy = 0
-                    const T eta = T(d.eta), zeta = T(d.zeta);
-                    const cx<T> pxx = mul_mi((T(2) * eta * kx - (eta - zeta) * kx) * v);
-                    const cx<T> pxy = mul_mi((eta * ky) * v);
-                    const cx<T> pyy = mul_mi((-(eta - zeta) * kx) * v);
-                    acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
-                    acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
-                    if (pi_slot) {
-                        pi_slot[at] = pi_slot[at] + fw * pxx;
-                        pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
-                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
-                    }
-                    break;
-                }
-                case OUT_VY: {   // same with vx = 0
-                    const T eta = T(d.eta), zeta = T(d.zeta);
-                    const cx<T> pxx = mul_mi((-(eta - zeta) * ky) * v);
-                    const cx<T> pxy = mul_mi((eta * kx) * v);
-                    const cx<T> pyy = mul_mi((T(2) * eta * ky - (eta - zeta) * ky) * v);
-                    acc0[i] = acc0[i] + mul_mi(kx * pxx + ky * pxy);
-                    acc1[i] = acc1[i] + mul_mi(kx * pxy + ky * pyy);
-                    if (pi_slot) {
-                        pi_slot[at] = pi_slot[at] + fw * pxx;
-                        pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
-                        pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
-                    }
-                    break;
-                }
-                case OUT_PXX:   // src/fcalculator.cc:472-491
-                    acc0[i] = acc0[i] + mul_mi(kx * v);
-                    if (pi_slot) pi_slot[at] = pi_slot[at] + fw * v;
-                    break;
-                case OUT_PXY:
-                    acc0[i] = acc0[i] + mul_mi(ky * v);
-                    acc1[i] = acc1[i] + mul_mi(kx * v);
-                    if (pi_slot) pi_slot[at + fe] = pi_slot[at + fe] + fw * v;
-                    break;
-                default:   // OUT_PYY
-                    acc1[i] = acc1[i] + mul_mi(ky * v);
-                    if (pi_slot) pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * v;
-                    break;
             }
         }
         __syncthreads();
@@ -536,7 +516,6 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     __syncthreads();
 
     // ---- spectral terms, noise, stage update --------------------------------
-    const bool psi_group = g.kind == GROUP_PSI;
     const int a = g.alpha;
     const bool compressible = d.mode == SHHD_MODE_COMPRESSIBLE;
     const int fx_field = d.N + 1, fy_field = d.N + 2;
@@ -561,8 +540,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             continue;
         }
         const int ky_i = cidx < half ? cidx : cidx - nay;
-        const T kx = T(two_pi * gx / d.lx);
-        const T ky = T(two_pi * ky_i / d.ly);
+        const T kx = T(d.dkx) * T(gx);
+        const T ky = T(d.dky) * T(ky_i);
         const T k2 = kx * kx + ky * ky;
         const bool is_k0 = kFp32 && gx == 0 && cidx == 0 && P.next_k0 != nullptr;
         // canonical mode of the noise draw: the kx = 0 line draws the ky > 0

@@ -14,6 +14,8 @@
 // FP64 on the host (and rounded for the FP32 form).
 #pragma once
 
+#include <utility>
+
 #include "cx.h"
 
 namespace shhd {
@@ -164,6 +166,129 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, int L, int nseq, const FftPl
         else if (ps.radix == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ps, tw, sign);
         else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ps, tw, sign);
     }
+}
+
+// ---------------------------------------------------------------------------
+// The same transform with the length fixed at compile time.  Every pass's
+// radix, stride and twiddle step are constants, so the sequence split is a
+// multiply by a constant, the LDS offsets of a butterfly's R values are
+// immediates (lp is linear over a stride that is a multiple of 16), and the
+// pass loop and radix dispatch disappear.  Used for the 3*2^k lengths of the
+// 3/2-padded grids; every other length takes the runtime plan above.
+// ---------------------------------------------------------------------------
+constexpr int ct_log2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+// radix of pass P of length L: the host plan's rule (8s, then 4, 2, then 3)
+constexpr int ct_radix(int L, int P) {
+    int m = L, p = 0;
+    while (m % 8 == 0) { if (p == P) return 8; m /= 8; ++p; }
+    while (m % 4 == 0) { if (p == P) return 4; m /= 4; ++p; }
+    while (m % 2 == 0) { if (p == P) return 2; m /= 2; ++p; }
+    while (m % 3 == 0) { if (p == P) return 3; m /= 3; ++p; }
+    return 0;
+}
+constexpr int ct_npass(int L) {
+    int n = 0;
+    while (ct_radix(L, n) != 0) ++n;
+    return n;
+}
+constexpr int ct_ns(int L, int P) {
+    int ns = 1;
+    for (int p = 0; p < P; ++p) ns *= ct_radix(L, p);
+    return ns;
+}
+
+template <typename T, int NT, int VPT, int L, int R, int NS>
+__device__ __forceinline__ void fft_pass_ct(cx<T>* buf, int nseq, const cx<T>* __restrict__ tw, int sign, int tid) {
+    constexpr int M = L / R;
+    constexpr int LNS = ct_log2(NS);
+    constexpr int TSTEP = L / (NS * R);
+    constexpr int B = (VPT + R - 1) / R;
+    const int nb = nseq * M;
+    cx<T> v[B][R];
+    int jj[B], sl[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int id = tid + NT * b;
+        jj[b] = 0;
+        sl[b] = 0;
+        if (id < nb) {
+            const int s = id / M;
+            const int j = id - s * M;
+            jj[b] = j;
+            sl[b] = s * L;
+            const int a = s * L + j;
+            if constexpr (M % 16 == 0) {
+                const int a0 = lp(a);
+#pragma unroll
+                for (int q = 0; q < R; ++q) v[b][q] = buf[a0 + q * (M + M / 16)];
+            } else {
+#pragma unroll
+                for (int q = 0; q < R; ++q) v[b][q] = buf[lp(a + q * M)];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int id = tid + NT * b;
+        if (id < nb) {
+            const int j = jj[b];
+            const int k = j & (NS - 1);
+            if constexpr (NS > 1) {
+                if (k != 0) apply_twiddles<T, R>(v[b], tw, k * TSTEP, sign);
+            }
+            dft_small<T, R>(v[b], sign);
+            const int out = sl[b] + ((j >> LNS) << LNS) * R + k;
+            if constexpr (NS % 16 == 0) {
+                const int o0 = lp(out);
+#pragma unroll
+                for (int p = 0; p < R; ++p) buf[o0 + p * (NS + NS / 16)] = v[b][p];
+            } else {
+#pragma unroll
+                for (int p = 0; p < R; ++p) buf[lp(out + p * NS)] = v[b][p];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int NT, int VPT, int L, int... P>
+__device__ __forceinline__ void lds_fft_ct_passes(cx<T>* buf, int nseq, const cx<T>* __restrict__ tw, int sign,
+                                                  int tid, std::integer_sequence<int, P...>) {
+    (fft_pass_ct<T, NT, VPT, L, ct_radix(L, P), ct_ns(L, P)>(buf, nseq, tw, sign, tid), ...);
+}
+
+template <typename T, int NT, int VPT, int L>
+__device__ __forceinline__ void lds_fft_ct(cx<T>* buf, int nseq, const cx<T>* __restrict__ tw, int sign) {
+    int tid = static_cast<int>(threadIdx.x);
+    // an opaque copy of the lane index: the constant-folded addresses of this
+    // call are then formed here, not hoisted out of the caller's loops and
+    // held live across them
+    asm volatile("" : "+v"(tid));
+    lds_fft_ct_passes<T, NT, VPT, L>(buf, nseq, tw, sign, tid, std::make_integer_sequence<int, ct_npass(L)>{});
+}
+
+// Fixed-length transform when L is one of the 3*2^k lengths that fit this
+// launch shape, else the runtime plan.
+template <typename T, int NT, int VPT>
+__device__ __forceinline__ void lds_fft_any(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
+                                            const cx<T>* __restrict__ tw, int sign) {
+    constexpr int CAP = NT * VPT;
+#define SHHD_FFT_CT(LEN)                                                  \
+    if constexpr (VPT == 6 && LEN <= CAP && 8 * LEN >= CAP) {             \
+        if (L == LEN) {                                                   \
+            lds_fft_ct<T, NT, VPT, LEN>(buf, nseq, tw, sign);             \
+            return;                                                       \
+        }                                                                 \
+    }
+    SHHD_FFT_CT(96) SHHD_FFT_CT(192) SHHD_FFT_CT(384) SHHD_FFT_CT(768)
+    SHHD_FFT_CT(1536) SHHD_FFT_CT(3072) SHHD_FFT_CT(6144)
+#undef SHHD_FFT_CT
+    lds_fft<T, NT, VPT>(buf, L, nseq, plan, tw, sign);
 }
 
 }  // namespace shhd

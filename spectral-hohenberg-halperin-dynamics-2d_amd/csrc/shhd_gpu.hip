@@ -334,6 +334,52 @@ struct Instance {
     }
 };
 
+// Coefficients of one product's spectral terms (ContribCoef in desc.h):
+//   mu_phys:   F = -i M k mu_hat,            rhs -= M k^2 mu_hat  (src/fcalculator.cc:244-256)
+//   psi v:     F = (psi v)_hat,               rhs -= i k . F       (258-272)
+//   gradient:  rhs += source (compressible force/gradient, 776-840); no flux
+//   v_x, v_y:  Pi = -i (2 eta k v - (eta - zeta)(k . v) 1), rhs -= i k . Pi  (343-398)
+//   j v:       Pi = (j v)_hat,                 rhs -= i k . Pi      (472-491)
+ContribCoef contrib_coef(const Desc& d, OutItem it) {
+    ContribCoef c{};
+    const double eta = d.eta, zeta = d.zeta;
+    switch (it.kind) {
+        case OUT_MU: {
+            const double M = d.mobility[it.alpha];
+            c.a[0][1] = -M;
+            c.a[0][2] = -M;
+            c.f[0][1] = M;        // F_x = -i M kx mu
+            c.f[1][2] = M;        // F_y = -i M ky mu
+            break;
+        }
+        case OUT_PSIVX: c.b[0][0] = 1.0; c.f[0][0] = 1.0; break;
+        case OUT_PSIVY: c.b[0][1] = 1.0; c.f[1][0] = 1.0; break;
+        case OUT_GRADF: c.a[0][0] = 1.0; break;
+        case OUT_VX:
+            // Pi_xx = -i (eta + zeta) kx v, Pi_xy = -i eta ky v, Pi_yy = i (eta - zeta) kx v
+            c.a[0][1] = -(eta + zeta);
+            c.a[0][2] = -eta;
+            c.a[1][3] = -zeta;
+            c.f[0][1] = eta + zeta;
+            c.f[1][2] = eta;
+            c.f[2][1] = -(eta - zeta);
+            break;
+        case OUT_VY:
+            // Pi_xx = i (eta - zeta) ky v, Pi_xy = -i eta kx v, Pi_yy = -i (eta + zeta) ky v
+            c.a[0][3] = -zeta;
+            c.a[1][1] = -eta;
+            c.a[1][2] = -(eta + zeta);
+            c.f[0][2] = -(eta - zeta);
+            c.f[1][1] = eta;
+            c.f[2][2] = eta + zeta;
+            break;
+        case OUT_PXX: c.b[0][0] = 1.0; c.f[0][0] = 1.0; break;
+        case OUT_PXY: c.b[0][1] = 1.0; c.b[1][0] = 1.0; c.f[1][0] = 1.0; break;
+        default: c.b[1][1] = 1.0; c.f[2][0] = 1.0; break;   // OUT_PYY
+    }
+    return c;
+}
+
 // Lower shhd_config into the kernel descriptor (see desc.h for the meaning of
 // each entry).  Mirrors the reference's request logic:
 // PhysicalRHSFieldRequests (src/fcalculator_field_requests_and_layout.h:30-127),
@@ -355,6 +401,8 @@ shhd_status build_desc(Instance* h) {
     d.hxa = c.nx_active / 2;
     d.lx = c.lx;
     d.ly = c.ly;
+    d.dkx = 2.0 * 3.14159265358979323846 / c.lx;
+    d.dky = 2.0 * 3.14159265358979323846 / c.ly;
     d.N = N;
     d.nfields = N + 3;
     d.mode = c.mode;
@@ -471,14 +519,20 @@ shhd_status build_desc(Instance* h) {
         g.kind = GROUP_PSI;
         g.alpha = a;
         g.ncontrib = static_cast<int>(psi_contrib[a].size());
-        for (int i = 0; i < g.ncontrib; ++i) g.contrib[i] = psi_contrib[a][static_cast<size_t>(i)];
+        for (int i = 0; i < g.ncontrib; ++i) {
+            g.contrib[i] = psi_contrib[a][static_cast<size_t>(i)];
+            g.coef[i] = contrib_coef(d, d.out[g.contrib[i]]);
+        }
     }
     if (!quiescent) {
         Group& g = d.groups[d.ngroups++];
         g.kind = GROUP_MOM;
         g.alpha = 0;
         g.ncontrib = static_cast<int>(mom_contrib.size());
-        for (int i = 0; i < g.ncontrib; ++i) g.contrib[i] = mom_contrib[static_cast<size_t>(i)];
+        for (int i = 0; i < g.ncontrib; ++i) {
+            g.contrib[i] = mom_contrib[static_cast<size_t>(i)];
+            g.coef[i] = contrib_coef(d, d.out[g.contrib[i]]);
+        }
     }
 
     // sine forces as constant spectral sources (src/fcalculator.cc:702-774)
