@@ -27,6 +27,17 @@
 
 namespace shhd {
 
+// XCD-aware workgroup order.  The dispatcher hands workgroup b to XCD b % 8,
+// and each XCD has its own L2; adjacent column tiles read and write
+// neighbouring segments of the same cache lines of the row-major T and U
+// planes.  Renumbering so that every XCD gets one contiguous range of tiles
+// lets its L2 serve those shared lines once instead of once per tile.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int X = 8;
+    const int x = b % X, i = b / X, q = nb / X, r = nb % X;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // signed ky index of padded row gy (src/domain.cc:145-149)
 __device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2) ? gy : gy - my; }
 
@@ -38,8 +49,9 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
                                                         const StagePtrs<T>* __restrict__ ptrs, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const int inst = map.inst[blockIdx.x];
-    const int local = blockIdx.x - map.first[inst];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int inst = map.inst[blk];
+    const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
@@ -383,8 +395,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                                                         BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const int inst = map.inst[blockIdx.x];
-    const int local = blockIdx.x - map.first[inst];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int inst = map.inst[blk];
+    const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
@@ -396,13 +409,16 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         sp.step_lo = static_cast<uint32_t>(st);
         sp.step_hi = static_cast<uint32_t>(st >> 32);
     }
-    const int group = local / I.ntiles;
+    // tile-major order: every XCD's range of workgroups holds all groups of its
+    // tiles, so the heavier momentum group does not pile up on a few XCDs
+    const int tile = local / d.ngroups;
+    const int group = local - tile * d.ngroups;
     const Group& g = d.groups[group];
     const int L = d.my;
     const int nay = d.nay;
     const int half = nay / 2;
     const int hxa = d.hxa;
-    const int gx0 = (local - group * I.ntiles) * C;
+    const int gx0 = tile * C;
     const int ncol = C * L;
     const int nmodes = C * nay;
     const size_t plane = static_cast<size_t>(L) * hxa;
