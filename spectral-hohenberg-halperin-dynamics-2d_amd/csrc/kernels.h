@@ -147,18 +147,20 @@ struct RowFields {
     }
 };
 
-// Row transforms of K2: the fixed-length transforms in FP32; in FP64 the
-// runtime plan, because the fixed-length code costs the FP64 row kernel the
-// few registers that separate 3 from 2 resident waves per SIMD (measured
-// slower at 1536^2).
-template <typename T, int NT, int PPT>
+// Row transforms of K2.  FL > 0: a row kernel built for rows of exactly FL
+// points, with only the fixed-length transform (no runtime-plan fallback
+// competing for registers).  FL = 0: any length; the fixed-length dispatch in
+// FP32, the runtime plan in FP64, where the dispatch costs the few registers
+// that separate 3 from 2 resident waves per SIMD.
+template <typename T, int NT, int PPT, int FL>
 __device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
                                         const cx<T>* __restrict__ tw, int sign) {
-    if constexpr (sizeof(T) == 4) lds_fft_any<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
+    if constexpr (FL > 0) lds_fft_ct<T, NT, PPT, FL>(buf, nseq, tw, sign);
+    else if constexpr (sizeof(T) == 4) lds_fft_any<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
     else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
 }
 
-template <typename T, int NT, int PPT, int MAXN>
+template <typename T, int NT, int PPT, int MAXN, int FL>
 __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             lds[lp(idx)] = z;
         }
         __syncthreads();
-        row_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, +1);
+        row_fft<T, NT, PPT, FL>(lds, L, R, d.plan_x, twx, +1);
         const int ra = d.in_role[s];
         const int rb = has_b ? d.in_role[s + 1] : -1;
 #pragma unroll
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             }
         }
         __syncthreads();
-        row_fft<T, NT, PPT>(lds, L, R, d.plan_x, twx, -1);
+        row_fft<T, NT, PPT, FL>(lds, L, R, d.plan_x, twx, -1);
         cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
         cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
         const int nk = R * hxa;
@@ -595,20 +597,29 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             }
             cx<T> sto = czero<T>();
             if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
-                const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
-                const cx<T> xa = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 0u, sp);
-                const cx<T> sa = sigma * xa;
-                cx<T> na = sa;
-                if (sp.use_b) {
-                    const cx<T> xb = gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 1u, sp);
-                    na = na + beta * (sigma * xb);
+                // draws A and B (conjugated on the kx = 0, ky < 0 half line);
+                // SRK3 draws them once per step: stage 1 stores them, stages 2
+                // and 3 read them back
+                cx<T> sa, sb = czero<T>();
+                cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
+                if (sp.noise_mode == NOISE_LOAD) {
+                    sa = nz[at];
+                    sb = nz[fe + at];
+                } else {
+                    const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
+                    sa = sigma * gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 0u, sp);
+                    if (sp.use_b) sb = sigma * gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 1u, sp);
+                    if (conj_draw) { sa = conj(sa); sb = conj(sb); }
+                    if (sp.noise_mode == NOISE_STORE) {
+                        nz[at] = sa;
+                        nz[fe + at] = sb;
+                    }
                 }
-                sto = conj_draw ? conj(na) : na;
+                sto = sp.use_b ? sa + beta * sb : sa;
                 if (fx_slot && sp.flux_first) {   // draw A: (i k / k^2) sigma xi, times 1/sqrt(dt)
-                    const cx<T> fa = conj_draw ? conj(sa) : sa;
                     const T isd = T(1.0 / d.sqrt_dt);
-                    fx_slot[at] = fx_slot[at] + isd * mul_i((kx / k2) * fa);
-                    fy_slot[at] = fy_slot[at] + isd * mul_i((ky / k2) * fa);
+                    fx_slot[at] = fx_slot[at] + isd * mul_i((kx / k2) * sa);
+                    fy_slot[at] = fy_slot[at] + isd * mul_i((ky / k2) * sa);
                 }
             }
             if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
@@ -651,7 +662,11 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
             }
             cx<T> sx = czero<T>(), sy = czero<T>();
-            if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
+            cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
+            if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
+                sx = nzj[at] + beta * nzj[2 * fe + at];
+                sy = nzj[fe + at] + beta * nzj[3 * fe + at];
+            } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
                 const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
                 const T kxd = conj_draw ? T(0) : kx;
                 const T kyd = conj_draw ? -ky : ky;
@@ -671,6 +686,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                         pi_slot[at] = pi_slot[at] + isd * (conj_draw ? conj(pxx) : pxx);
                         pi_slot[at + fe] = pi_slot[at + fe] + isd * (conj_draw ? conj(pxy) : pxy);
                         pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
+                    }
+                    if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
+                        nzj[static_cast<size_t>(2 * ab) * fe + at] = vx;
+                        nzj[static_cast<size_t>(2 * ab + 1) * fe + at] = vy;
                     }
                     if (ab == 0) { sx = vx; sy = vy; }
                     else { sx = sx + beta * vx; sy = sy + beta * vy; }

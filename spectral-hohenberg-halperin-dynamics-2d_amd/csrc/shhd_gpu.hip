@@ -52,6 +52,7 @@ struct ColPlan {
 };
 struct RowPlan {
     int R = 1, NT = 256, V = 6, nblocks = 1, maxn = 0;
+    int fl = 0;        // row length of a fixed-length row-kernel build, 0 for the any-length one
     size_t lds = 0;
 };
 
@@ -119,6 +120,8 @@ ColPlan plan_columns(int my, int nay, int hxa, size_t elem, int nt_min) {
     return p;
 }
 
+int row_fixed_length(int mx, int NT, int V);
+
 RowPlan plan_rows(int mx, int my, size_t elem, int nt_min) {
     RowPlan p;
     p.V = (mx % 3 == 0) ? 6 : 8;
@@ -127,6 +130,7 @@ RowPlan plan_rows(int mx, int my, size_t elem, int nt_min) {
     while (p.R > 1 && static_cast<size_t>(lp_size(p.R * mx)) * elem > kLdsCap) --p.R;
     p.R = std::min(p.R, my);
     p.nblocks = (my + p.R - 1) / p.R;
+    p.fl = row_fixed_length(mx, p.NT, p.V);
     p.lds = static_cast<size_t>(lp_size(p.R * mx)) * elem;
     return p;
 }
@@ -159,10 +163,26 @@ K3Fn<T> pick_k3(int NT, int V, int acc) {
     return nullptr;
 }
 
+// row length with its own row-kernel build for this lane shape, else 0
+int row_fixed_length(int mx, int NT, int V) {
+#define SHHD_RFL(nt, v, len) if (mx == len && NT == nt && V == v) return len;
+    SHHD_FOR_ROW_FL(SHHD_RFL)
+#undef SHHD_RFL
+    return 0;
+}
+
 template <typename T>
-K2Fn<T> pick_k2(int NT, int V, int maxn) {
+K2Fn<T> pick_k2(int NT, int V, int maxn, int fl) {
     const int n = maxn <= 0 ? 0 : (maxn == 1 ? 1 : (maxn == 2 ? 2 : (maxn <= 4 ? 4 : 8)));
-#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn>;
+    if (fl > 0) {
+#define SHHD_K2FN(nt, v, mn, len) if (NT == nt && V == v && n == mn && fl == len) return &k_rows<T, nt, v, mn, len>;
+#define SHHD_K2F(nt, v, len) SHHD_K2FN(nt, v, 0, len) SHHD_K2FN(nt, v, 1, len) SHHD_K2FN(nt, v, 2, len) \
+    SHHD_K2FN(nt, v, 4, len) SHHD_K2FN(nt, v, 8, len)
+        SHHD_FOR_ROW_FL(SHHD_K2F)
+#undef SHHD_K2F
+#undef SHHD_K2FN
+    }
+#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0>;
 #define SHHD_K2(nt, v) SHHD_K2N(nt, v, 0) SHHD_K2N(nt, v, 1) SHHD_K2N(nt, v, 2) SHHD_K2N(nt, v, 4) SHHD_K2N(nt, v, 8)
     SHHD_FOR_NT_V(SHHD_K2)
 #undef SHHD_K2
@@ -272,6 +292,7 @@ struct Instance {
     cx<double>* k0[3] = {nullptr, nullptr, nullptr};
     void* tbuf = nullptr;
     void* ubuf = nullptr;
+    void* noise = nullptr;    // SRK3 noise of the current step (NoiseMode)
     void* twx = nullptr;
     void* twy = nullptr;
     std::vector<int> static_fields;
@@ -329,6 +350,7 @@ struct Instance {
         }
         if (tbuf) (void)hipFree(tbuf);
         if (ubuf) (void)hipFree(ubuf);
+        if (noise) (void)hipFree(noise);
         if (twx) (void)hipFree(twx);
         if (twy) (void)hipFree(twy);
     }
@@ -676,7 +698,17 @@ shhd_status validate(const shhd_config& c) {
 }
 
 
-// bytes a configuration holds on the device (state x3, work planes, twiddles)
+// SRK3 with noise keeps the step's draws A and B for stages 2 and 3:
+// 2 fields per order parameter and 4 for the momentum
+size_t noise_elems(const Instance& h) {
+    const Desc& d = h.desc;
+    bool on = d.j_noise_on != 0;
+    for (int a = 0; a < d.N; ++a) on = on || d.psi_noise_on[a] != 0;
+    if (h.stages != 3 || !on) return 0;
+    return static_cast<size_t>(2 * d.N + 4) * h.field_elems;
+}
+
+// bytes a configuration holds on the device (state x3, work planes, noise, twiddles)
 size_t footprint_of(const Instance& h) {
     const size_t nf = static_cast<size_t>(h.desc.nfields);
     size_t b = 3 * nf * h.field_elems * h.elem;
@@ -684,6 +716,7 @@ size_t footprint_of(const Instance& h) {
     b += std::max<size_t>(1, static_cast<size_t>(h.n_in)) * h.plane_elems * h.elem;
     b += std::max<size_t>(1, static_cast<size_t>(h.n_out)) * h.plane_elems * h.elem;
     b += static_cast<size_t>(h.cfg.nx + h.cfg.ny) * h.elem;
+    b += noise_elems(h) * h.elem;
     return b;
 }
 
@@ -737,6 +770,7 @@ shhd_status allocate(Instance* h, std::string* err) {
     }
     SHHD_HIP(err, hipMalloc(&h->tbuf, std::max<size_t>(1, static_cast<size_t>(h->n_in)) * h->plane_elems * h->elem));
     SHHD_HIP(err, hipMalloc(&h->ubuf, std::max<size_t>(1, static_cast<size_t>(h->n_out)) * h->plane_elems * h->elem));
+    if (const size_t ne = noise_elems(*h)) SHHD_HIP(err, hipMalloc(&h->noise, ne * h->elem));
     return h->fp32 ? upload_twiddles<float>(h, err) : upload_twiddles<double>(h, err);
 }
 
@@ -845,10 +879,10 @@ shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t
 // ---------------------------------------------------------------------------
 struct ClassKey {
     bool fp32;
-    int stages, cNT, cV, acc, rNT, rV, maxn;
+    int stages, cNT, cV, acc, rNT, rV, maxn, rfl;
     bool operator<(const ClassKey& o) const {
-        return std::tie(fp32, stages, cNT, cV, acc, rNT, rV, maxn) <
-               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.acc, o.rNT, o.rV, o.maxn);
+        return std::tie(fp32, stages, cNT, cV, acc, rNT, rV, maxn, rfl) <
+               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.acc, o.rNT, o.rV, o.maxn, o.rfl);
     }
 };
 
@@ -1427,6 +1461,7 @@ struct Engine {
             I.flag = d_flags + lc.members[static_cast<size_t>(i)];
             I.step = d_steps + lc.members[static_cast<size_t>(i)];
             I.flux = h.flux_req ? static_cast<cx<T>*>(h.flux) : nullptr;
+            I.noise = static_cast<cx<T>*>(h.noise);
             I.flux_req = h.flux_req;
             I.C = h.cols.C;
             I.R = h.rows.R;
@@ -1517,7 +1552,7 @@ struct Engine {
         const ClassKey& K = lc.key;
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.cNT, K.cV, K.acc)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
@@ -1533,7 +1568,8 @@ struct Engine {
         for (int s = 0; s < kMaxSlots; ++s) {
             const Instance* h = slots[static_cast<size_t>(s)].get();
             if (!h) continue;
-            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols.acc, h->rows.NT, h->rows.V, h->rows.maxn};
+            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols.acc, h->rows.NT, h->rows.V, h->rows.maxn,
+                       h->rows.fl};
             groups[k].push_back(s);
         }
         for (auto& g : groups) {
@@ -1594,7 +1630,7 @@ struct Engine {
             if (st != SHHD_OK) return st;
         }
         if (lc.nblk[1] > 0) {
-            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn);
+            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl);
             st = timed(1, capturing, [&] {
                 hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], stream, inst, lc.map[1]);
             });
@@ -1644,6 +1680,7 @@ struct Engine {
                     sp.flux_w = 1.0;
                 }
                 sp.flux_first = s == 0 ? 1 : 0;
+                sp.noise_mode = lc.key.stages == 3 ? (s == 0 ? NOISE_STORE : NOISE_LOAD) : NOISE_DRAW;
                 sp.flux_last = s == lc.key.stages - 1 ? 1 : 0;
                 shhd_status st = lc.key.fp32 ? launch_stage<float>(lc, k, s, sp, capturing)
                                              : launch_stage<double>(lc, k, s, sp, capturing);
