@@ -189,6 +189,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form side measurement")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
@@ -209,6 +210,8 @@ def main():
         sys.exit(2)
 
     cfg, desc = workload(args.workload, args.precision)
+    # launch shapes for this problem shape, measured before the timed region
+    tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
     el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup)
     pts = cfg.nx_active * cfg.ny_active
     value = pts * args.steps / el / 1e6
@@ -246,7 +249,8 @@ def main():
         "config": {"workload": desc, "grid": [cfg.nx_active, cfg.ny_active], "padded": [cfg.nx, cfg.ny],
                    "order_parameters": cfg.num_order_parameters, "integrator": "srk3",
                    "F_in": info.num_inverse_fields, "F_out": info.num_forward_fields,
-                   "F_s": info.num_updated_fields, "parallelism": "single GPU"},
+                   "F_s": info.num_updated_fields, "parallelism": "single GPU",
+                   "launch_shapes": tuning},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,
@@ -254,9 +258,11 @@ def main():
     }
     if args.precision == 64 and not args.no_fp32:
         cfg32, _ = workload(args.workload, 32)
+        tuning32 = shhd.autotune(cfg32, steps=4) if not args.no_tune else None
         el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup)
         v32 = pts * args.steps / el32 / 1e6
-        result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value}
+        result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value,
+                          "launch_shapes": tuning32}
     if not args.no_serve:
         result["ensemble"] = ensemble_bench(args.ensemble)
         result["served"] = served_bench(args.serve_jobs)

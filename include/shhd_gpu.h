@@ -268,6 +268,22 @@ shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info);
 shhd_status shhd_gpu_set_kernel_timing(shhd_gpu* h, int enable);
 shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long* launches, int nclasses);
 
+/* ---- launch-shape autotuner (no reference counterpart) ------------------------
+ * Times the candidate column tiles of the inverse and forward column passes
+ * and the candidate lane counts of the row pass for this configuration's
+ * problem shape on a scratch instance (`steps` timed steps per candidate),
+ * and records the fastest of each for the process: every handle or batch
+ * instance of that shape created afterwards uses them.  Results are the same
+ * whatever the shapes (only the workgroup decomposition changes). */
+typedef struct shhd_tuning {
+    int inverse_column_tile;   /* kx columns per inverse-column workgroup     */
+    int forward_column_tile;   /* kx columns per forward-column workgroup     */
+    int row_lanes;             /* lanes per row-pass workgroup                */
+    double ms[3];              /* mean launch time of each kernel class (as
+                                  shhd_gpu_get_kernel_timing) with that shape */
+} shhd_tuning;
+shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out);
+
 /* Algorithmic HBM bytes moved by one launch of each kernel class (the byte
  * model: compact spectra s*Na^2 per field, column/row intermediates
  * 1.5*s*Na^2 per field under 3/2 padding). */

@@ -170,9 +170,11 @@ struct InstDev {
     int flux_req;             // FluxBits
     cx<T>* noise;             // SRK3 noise of the step: per psi group draws A, B;
                               // momentum: A_x, A_y, B_x, B_y (fields of field_elems); null if no noise
-    int C;                    // kx columns per column-pass workgroup
+    int C;                    // kx columns per inverse-column workgroup
     int R;                    // rows per row-pass workgroup
-    int ntiles;               // column tiles = ceil(Nax/2 / C)
+    int ntiles;               // inverse-column tiles = ceil(Nax/2 / C)
+    int C3;                   // kx columns per forward-column workgroup
+    int ntiles3;              // forward-column tiles
 };
 
 struct BlockMap {

@@ -403,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     const InstDev<T>& I = insts[inst];
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
-    const int C = I.C;
+    const int C = I.C3;
     const cx<T>* __restrict__ ubuf = I.ubuf;
     const cx<T>* __restrict__ twy = I.twy;
     {

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -99,14 +100,14 @@ int lanes_for(int n, int v, int nt_min) {
     return std::max(nt, nt_min);
 }
 
-// Column tile of C kx columns per workgroup.  The natural plan targets
-// 32-byte row segments; a batch plan (nt_min = 256) widens C so small grids
-// share the 256-lane shape of the larger ones.
-ColPlan plan_columns(int my, int nay, int hxa, size_t elem, int nt_min) {
+// Column tile of C kx columns per workgroup: `c_tile` when the tuner chose
+// one, else the tile whose row segments are `seg_bytes` wide; a batch plan
+// (nt_min = 256) widens C so small grids share the 256-lane shape of the
+// larger ones.  The inverse and forward column passes are planned separately.
+ColPlan plan_columns(int my, int nay, int hxa, size_t elem, int nt_min, int seg_bytes, int c_tile) {
     ColPlan p;
     p.V = (my % 3 == 0) ? 6 : 8;
-    int target = static_cast<int>(32 / elem);   // 32-byte row segments (measured best at 1536^2)
-    if (const char* e = std::getenv("SHHD_COLUMN_TILE")) target = std::max(1, std::atoi(e));   // tuning override
+    int target = c_tile > 0 ? c_tile : std::max(1, static_cast<int>(seg_bytes / static_cast<int>(elem)));
     if (nt_min > 0) target = std::max(target, (nt_min * p.V) / my);
     int C = std::min(std::max(target, 1), pow2ceil(hxa));
     auto lds3 = [&](int c) { return static_cast<size_t>(std::max(lp_size(c * my), 2 * c * nay)) * elem; };
@@ -283,7 +284,8 @@ struct Instance {
     bool fp32 = false;
     size_t elem = 16;
     int stages = 3;
-    ColPlan cols;
+    ColPlan cols;     // inverse-column pass (K1, physical export)
+    ColPlan cols3;    // forward-column pass (K3)
     RowPlan rows;
     int n_in = 0, n_out = 0, n_updated = 0;
     size_t field_elems = 0;   // hxa * nay
@@ -301,6 +303,8 @@ struct Instance {
     int cur = 0;
     uint64_t step_index = 0;
     size_t bytes = 0;
+    bool batch_shape = false;
+    std::string tune_key;
     // flux recording (FluxBuffer, src/buffer_flux.h): slots of field_elems each
     void* flux = nullptr;
     int flux_req = 0;           // FluxBits
@@ -698,6 +702,60 @@ shhd_status validate(const shhd_config& c) {
 }
 
 
+// ---------------------------------------------------------------------------
+// launch-shape tuning: the column tiles of the two column passes and the lanes
+// of the row pass, measured by shhd_autotune for a shape of problem and kept
+// for the process (every later instance of that shape uses them)
+// ---------------------------------------------------------------------------
+struct Tuning {
+    int c1 = 0, c3 = 0, rnt = 0;   // 0: the untuned default
+};
+std::mutex g_tune_mutex;
+std::map<std::string, Tuning> g_tuned;
+
+// what the launch shapes depend on: precision, grids, transform counts,
+// groups, stages, mode, order parameters held in the row pass, noise, batch plan
+std::string tune_key_of(const Instance& h) {
+    const Desc& d = h.desc;
+    bool noise = d.j_noise_on != 0;
+    for (int a = 0; a < d.N; ++a) noise = noise || d.psi_noise_on[a] != 0;
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "fp%d %dx%d a%dx%d in%d out%d g%d s%d m%d n%d z%d b%d", h.fp32 ? 32 : 64, d.mx,
+                  d.my, d.nax, d.nay, d.n_in, d.n_out, d.ngroups, h.stages, d.mode, d.N, noise ? 1 : 0,
+                  h.batch_shape ? 1 : 0);
+    return buf;
+}
+
+bool lookup_tuning(const std::string& key, Tuning& t) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) return false;
+    t = it->second;
+    return true;
+}
+
+void store_tuning(const std::string& key, const Tuning& t) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[key] = t;
+}
+
+RowPlan plan_rows(int mx, int my, size_t elem, int nt_min);
+
+// The three plans of an instance for a tuning (0 entries: defaults: 32-byte
+// tiles for the inverse column pass, 16-byte tiles for the forward one, which
+// holds more per-mode state and runs more workgroups per CU with narrow tiles).
+void apply_plans(Instance& h, const Tuning& t) {
+    const Desc& d = h.desc;
+    const int nt_min = h.batch_shape ? 256 : 0;
+    h.cols = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 32, t.c1);
+    h.cols3 = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 16, t.c3);
+    h.rows = plan_rows(d.mx, d.my, h.elem, std::max(nt_min, t.rnt));
+    int npsi_roles = 0;
+    for (int i = 0; i < d.n_in; ++i)
+        if (d.in_role[i] >= ROLE_PSI0) npsi_roles = std::max(npsi_roles, d.in_role[i] - ROLE_PSI0 + 1);
+    h.rows.maxn = npsi_roles == 0 ? 0 : (npsi_roles == 1 ? 1 : (npsi_roles == 2 ? 2 : (npsi_roles <= 4 ? 4 : 8)));
+}
+
 // SRK3 with noise keeps the step's draws A and B for stages 2 and 3:
 // 2 fields per order parameter and 4 for the momentum
 size_t noise_elems(const Instance& h) {
@@ -721,7 +779,8 @@ size_t footprint_of(const Instance& h) {
 }
 
 // Lower and plan an instance (no device memory yet).
-shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<Instance>& out) {
+shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<Instance>& out,
+                    const Tuning* force = nullptr) {
     shhd_status st = validate(cfg);
     if (st != SHHD_OK) return st;
     auto h = std::make_unique<Instance>();
@@ -732,13 +791,12 @@ shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<In
     st = build_desc(h.get());
     if (st != SHHD_OK) return st;
     const Desc& d = h->desc;
-    const int nt_min = batch_shape ? 256 : 0;
-    h->cols = plan_columns(d.my, d.nay, d.hxa, h->elem, nt_min);
-    h->rows = plan_rows(d.mx, d.my, h->elem, nt_min);
-    int npsi_roles = 0;
-    for (int i = 0; i < d.n_in; ++i)
-        if (d.in_role[i] >= ROLE_PSI0) npsi_roles = std::max(npsi_roles, d.in_role[i] - ROLE_PSI0 + 1);
-    h->rows.maxn = npsi_roles == 0 ? 0 : (npsi_roles == 1 ? 1 : (npsi_roles == 2 ? 2 : (npsi_roles <= 4 ? 4 : 8)));
+    h->batch_shape = batch_shape;
+    h->tune_key = tune_key_of(*h);
+    Tuning t{};
+    if (force) t = *force;
+    else lookup_tuning(h->tune_key, t);
+    apply_plans(*h, t);
     h->field_elems = static_cast<size_t>(d.hxa) * d.nay;
     h->plane_elems = static_cast<size_t>(d.my) * d.hxa;
     h->bytes = footprint_of(*h);
@@ -879,10 +937,10 @@ shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t
 // ---------------------------------------------------------------------------
 struct ClassKey {
     bool fp32;
-    int stages, cNT, cV, acc, rNT, rV, maxn, rfl;
+    int stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl;
     bool operator<(const ClassKey& o) const {
-        return std::tie(fp32, stages, cNT, cV, acc, rNT, rV, maxn, rfl) <
-               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.acc, o.rNT, o.rV, o.maxn, o.rfl);
+        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl) <
+               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.c3NT, o.c3V, o.acc, o.rNT, o.rV, o.maxn, o.rfl);
     }
 };
 
@@ -1017,6 +1075,16 @@ struct Engine {
     Instance* at(int slot) {
         if (slot < 0 || slot >= kMaxSlots) return nullptr;
         return slots[static_cast<size_t>(slot)].get();
+    }
+
+    // new launch shapes for an instance (the buffers do not depend on them)
+    shhd_status replan(int slot, const Tuning& t) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        SHHD_HIP(&err, hipStreamSynchronize(stream));
+        apply_plans(*h, t);
+        dirty = true;
+        return SHHD_OK;
     }
 
     // FluidSolver::set_flux_request (src/fluid_solver.h:101-103).  The buffer
@@ -1466,6 +1534,8 @@ struct Engine {
             I.C = h.cols.C;
             I.R = h.rows.R;
             I.ntiles = h.cols.ntiles;
+            I.C3 = h.cols3.C;
+            I.ntiles3 = h.cols3.ntiles;
             // stage pointers for the k-th step after this build, rotating the
             // three buffers without copies: SRK3 stages (next, cur, base) =
             // (y, x, x), (z, y, x), (y, z, x); Euler (y, x, x); then cur <- y
@@ -1477,7 +1547,7 @@ struct Engine {
             }
             const int nb[3] = {h.desc.n_in > 0 ? h.cols.ntiles * h.desc.n_in : 0,
                                (h.desc.n_in > 0 || h.desc.n_out > 0) ? h.rows.nblocks : 0,
-                               h.cols.ntiles * h.desc.ngroups};
+                               h.cols3.ntiles * h.desc.ngroups};
             for (int kk = 0; kk < 3; ++kk) {
                 firsts[kk][static_cast<size_t>(i)] = static_cast<int>(maps[kk].size());
                 for (int b = 0; b < nb[kk]; ++b) maps[kk].push_back(i);
@@ -1487,7 +1557,7 @@ struct Engine {
                                    h.fp32 ? h.k0[0] + f : nullptr, h.desc.nay});
             lc.lds[0] = std::max(lc.lds[0], h.cols.lds);
             lc.lds[1] = std::max(lc.lds[1], h.rows.lds);
-            lc.lds[2] = std::max(lc.lds[2], h.cols.lds3);
+            lc.lds[2] = std::max(lc.lds[2], h.cols3.lds3);
         }
         SHHD_HIP(&err, hipMalloc(&lc.d_inst, inst.size() * sizeof(InstDev<T>)));
         SHHD_HIP(&err, hipMemcpy(lc.d_inst, inst.data(), inst.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
@@ -1554,7 +1624,7 @@ struct Engine {
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.cNT, K.cV, K.acc)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
         return SHHD_OK;
     }
@@ -1568,8 +1638,8 @@ struct Engine {
         for (int s = 0; s < kMaxSlots; ++s) {
             const Instance* h = slots[static_cast<size_t>(s)].get();
             if (!h) continue;
-            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols.acc, h->rows.NT, h->rows.V, h->rows.maxn,
-                       h->rows.fl};
+            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols3.NT, h->cols3.V, h->cols3.acc, h->rows.NT,
+                       h->rows.V, h->rows.maxn, h->rows.fl};
             groups[k].push_back(s);
         }
         for (auto& g : groups) {
@@ -1636,9 +1706,9 @@ struct Engine {
             });
             if (st != SHHD_OK) return st;
         }
-        K3Fn<T> k3 = pick_k3<T>(K.cNT, K.cV, K.acc);
+        K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc);
         return timed(2, capturing, [&] {
-            hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.cNT), lc.lds[2], stream, inst, ptrs, sp, lc.map[2]);
+            hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.c3NT), lc.lds[2], stream, inst, ptrs, sp, lc.map[2]);
         });
     }
 
@@ -1787,6 +1857,96 @@ struct Engine {
     }
 };
 
+// ---------------------------------------------------------------------------
+// autotuner: for one problem shape, time every candidate tile of the two
+// column passes and the candidate row-pass lane counts on a scratch instance
+// (uniform density, the configuration's own noise and terms), keep the fastest
+// of each kernel class and record it for the process
+// ---------------------------------------------------------------------------
+shhd_status time_classes(Engine& e, int steps, double ms[kClasses]) {
+    shhd_status st = e.step(2);   // builds the tables and graphs of the shape
+    if (st != SHHD_OK) return st;
+    e.timing = true;
+    for (int k = 0; k < kClasses; ++k) {
+        e.time_ms[k] = 0.0;
+        e.launches[k] = 0;
+    }
+    st = e.step(steps);
+    e.timing = false;
+    for (int k = 0; k < kClasses; ++k) ms[k] = e.launches[k] ? e.time_ms[k] / static_cast<double>(e.launches[k]) : 0.0;
+    return st;
+}
+
+shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, double best_ms[kClasses],
+                            std::string* err) {
+    Engine e;
+    shhd_status st = e.init(0, false, false);
+    if (st != SHHD_OK) return set_err(err, st, e.err);
+    int slot = -1;
+    st = e.admit(cfg, nullptr, 0, &slot);
+    if (st != SHHD_OK) return set_err(err, st, e.err);
+    shhd_initial_condition rho{};
+    rho.kind = SHHD_IC_EQUILIBRIUM_DENSITY;
+    rho.value = 1.0;
+    rho.pressure_coefficient = 1.0;
+    st = e.apply_initial_condition(slot, &rho);
+    if (st != SHHD_OK) return set_err(err, st, e.err);
+    Instance& h = *e.at(slot);
+    const Desc& d = h.desc;
+
+    // distinct candidate plans of each pass
+    auto col_candidates = [&](int seg) {
+        std::vector<int> out;
+        std::vector<int> seen;
+        for (int c : {1, 2, 4, 8}) {
+            const ColPlan p = plan_columns(d.my, d.nay, d.hxa, h.elem, 0, seg, c);
+            if (std::find(seen.begin(), seen.end(), p.C) != seen.end()) continue;
+            seen.push_back(p.C);
+            out.push_back(p.C);
+        }
+        return out;
+    };
+    std::vector<int> row_nts;
+    {
+        const RowPlan p0 = plan_rows(d.mx, d.my, h.elem, 0);
+        row_nts.push_back(p0.NT);
+        if (p0.NT < 1024) {
+            const RowPlan p1 = plan_rows(d.mx, d.my, h.elem, p0.NT == 64 ? 256 : 2 * p0.NT);
+            if (p1.NT != p0.NT && p1.R > p0.R) row_nts.push_back(p1.NT);
+        }
+    }
+    Tuning cur{};
+    double ms[kClasses];
+    // inverse column tile
+    double bk = 1e30;
+    for (int c : col_candidates(32)) {
+        Tuning t = cur;
+        t.c1 = c;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (d.n_in > 0 && ms[0] < bk) { bk = ms[0]; best.c1 = c; best_ms[0] = ms[0]; }
+    }
+    cur.c1 = best.c1;
+    // forward column tile
+    bk = 1e30;
+    for (int c : col_candidates(16)) {
+        Tuning t = cur;
+        t.c3 = c;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[2] < bk) { bk = ms[2]; best.c3 = c; best_ms[2] = ms[2]; }
+    }
+    cur.c3 = best.c3;
+    // row-pass lanes
+    bk = 1e30;
+    for (int nt : row_nts) {
+        Tuning t = cur;
+        t.rnt = nt;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[1] < bk) { bk = ms[1]; best.rnt = nt; best_ms[1] = ms[1]; }
+    }
+    store_tuning(h.tune_key, best);
+    return SHHD_OK;
+}
+
 double kernel_bytes_of(const Instance& h, int cls) {
     const Desc& d = h.desc;
     const double fb = static_cast<double>(h.field_elems) * static_cast<double>(h.elem);
@@ -1842,13 +2002,27 @@ shhd_status shhd_gpu_create(const shhd_config* config, shhd_gpu** out) {
         const Instance& I = *h->inst();
         const Desc& d = I.desc;
         std::fprintf(stderr,
-                     "[shhd] Mx=%d My=%d Na=%dx%d N=%d fp%d F_in=%d F_out=%d groups=%d | cols C=%d NT=%d V=%d acc=%d "
-                     "tiles=%d lds=%zu/%zu | rows R=%d NT=%d V=%d blocks=%d maxn=%d lds=%zu\n",
+                     "[shhd] Mx=%d My=%d Na=%dx%d N=%d fp%d F_in=%d F_out=%d groups=%d | cols C=%d NT=%d V=%d "
+                     "tiles=%d lds=%zu | cols3 C=%d NT=%d acc=%d lds=%zu | rows R=%d NT=%d V=%d blocks=%d maxn=%d lds=%zu\n",
                      d.mx, d.my, d.nax, d.nay, d.N, I.fp32 ? 32 : 64, d.n_in, d.n_out, d.ngroups, I.cols.C, I.cols.NT,
-                     I.cols.V, I.cols.acc, I.cols.ntiles, I.cols.lds, I.cols.lds3, I.rows.R, I.rows.NT, I.rows.V,
-                     I.rows.nblocks, I.rows.maxn, I.rows.lds);
+                     I.cols.V, I.cols.ntiles, I.cols.lds, I.cols3.C, I.cols3.NT, I.cols3.acc, I.cols3.lds3, I.rows.R,
+                     I.rows.NT, I.rows.V, I.rows.nblocks, I.rows.maxn, I.rows.lds);
     }
     *out = h.release();
+    return SHHD_OK;
+}
+
+shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out) {
+    if (!config || !out || steps <= 0) return set_err(nullptr, SHHD_ERR_INVALID, "invalid autotune arguments");
+    Tuning best{};
+    double ms[kClasses] = {0, 0, 0};
+    std::string err;
+    const shhd_status st = autotune_config(*config, steps, best, ms, &err);
+    if (st != SHHD_OK) return set_err(nullptr, st, err);
+    out->inverse_column_tile = best.c1;
+    out->forward_column_tile = best.c3;
+    out->row_lanes = best.rnt;
+    for (int k = 0; k < kClasses; ++k) out->ms[k] = ms[k];
     return SHHD_OK;
 }
 

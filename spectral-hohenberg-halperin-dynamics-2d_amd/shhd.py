@@ -50,6 +50,11 @@ PROBE_STATE, PROBE_FLUX = 0, 1
  IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE, IC_RANDOM_VORTICITY, IC_SPECTRUM_K4) = range(6)
 
 
+class ShhdTuning(ctypes.Structure):
+    _fields_ = [("inverse_column_tile", ctypes.c_int), ("forward_column_tile", ctypes.c_int),
+                ("row_lanes", ctypes.c_int), ("ms", ctypes.c_double * 3)]
+
+
 class ShhdInitialCondition(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("component", ctypes.c_int), ("seed", ctypes.c_int),
                 ("kBT", ctypes.c_double), ("value", ctypes.c_double), ("pressure_coefficient", ctypes.c_double),
@@ -288,6 +293,8 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_autotune.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.c_int, ctypes.POINTER(ShhdTuning)]
+    lib.shhd_autotune.restype = ctypes.c_int
     lib.shhd_write_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.shhd_write_snapshot.restype = ctypes.c_int
@@ -661,3 +668,17 @@ def read_snapshot(path: str):
     vals = table[:, 2:2 + 2 * nf]
     data = (vals[:, 0::2] + 1j * vals[:, 1::2]).T.reshape(nf, ny, nxs)
     return SNAPSHOT_SPECTRAL, step, time_, data
+
+
+def autotune(cfg: SolverConfig, steps: int = 6) -> dict:
+    """Launch-shape autotuner (shhd_autotune): measures the candidate column
+    tiles and row-pass lane counts for this problem shape and records the
+    fastest for every later solver or batch instance of the shape."""
+    lib = load_library()
+    c = cfg.to_c()
+    t = ShhdTuning()
+    st = lib.shhd_autotune(ctypes.byref(c), int(steps), ctypes.byref(t))
+    if st != 0:
+        raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())
+    return {"inverse_column_tile": t.inverse_column_tile, "forward_column_tile": t.forward_column_tile,
+            "row_lanes": t.row_lanes, "ms": list(t.ms)}
