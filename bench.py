@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
+    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT",
+                    help="use these launch shapes (inverse tile, forward tile, row lanes) instead of tuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
@@ -211,7 +213,12 @@ def main():
 
     cfg, desc = workload(args.workload, args.precision)
     # launch shapes for this problem shape, measured before the timed region
-    tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
+    if args.launch_shapes:
+        c1, c3, rnt = (int(v) for v in args.launch_shapes.split(","))
+        shhd.set_launch_shapes(cfg, c1, c3, rnt)
+        tuning = {"inverse_column_tile": c1, "forward_column_tile": c3, "row_lanes": rnt, "given": True}
+    else:
+        tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
     el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup)
     pts = cfg.nx_active * cfg.ny_active
     value = pts * args.steps / el / 1e6

@@ -283,6 +283,8 @@ typedef struct shhd_tuning {
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;
 shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out);
+/* Record launch shapes measured earlier (ms ignored; 0 entries: default). */
+shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* tuning);
 
 /* Algorithmic HBM bytes moved by one launch of each kernel class (the byte
  * model: compact spectra s*Na^2 per field, column/row intermediates

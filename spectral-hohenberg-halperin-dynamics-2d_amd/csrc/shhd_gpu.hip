@@ -2026,6 +2026,19 @@ shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out
     return SHHD_OK;
 }
 
+shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* t) {
+    if (!config || !t) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
+    std::unique_ptr<Instance> h;
+    const Tuning none{};
+    shhd_status st = prepare(*config, false, h, &none);
+    if (st != SHHD_OK) return st;
+    if (t->inverse_column_tile < 0 || t->forward_column_tile < 0 ||
+        (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024))
+        return set_err(nullptr, SHHD_ERR_INVALID, "invalid launch shapes");
+    store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes});
+    return SHHD_OK;
+}
+
 shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* in, size_t n) {
     if (!h || !in) return set_err(h ? &h->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
     Instance* I = h->inst();

@@ -295,6 +295,8 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_step.restype = ctypes.c_int
     lib.shhd_autotune.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.c_int, ctypes.POINTER(ShhdTuning)]
     lib.shhd_autotune.restype = ctypes.c_int
+    lib.shhd_set_tuning.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.POINTER(ShhdTuning)]
+    lib.shhd_set_tuning.restype = ctypes.c_int
     lib.shhd_write_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.shhd_write_snapshot.restype = ctypes.c_int
@@ -682,3 +684,13 @@ def autotune(cfg: SolverConfig, steps: int = 6) -> dict:
         raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())
     return {"inverse_column_tile": t.inverse_column_tile, "forward_column_tile": t.forward_column_tile,
             "row_lanes": t.row_lanes, "ms": list(t.ms)}
+
+
+def set_launch_shapes(cfg: SolverConfig, inverse_column_tile: int, forward_column_tile: int, row_lanes: int):
+    """Record launch shapes measured by an earlier autotune (shhd_set_tuning)."""
+    lib = load_library()
+    c = cfg.to_c()
+    t = ShhdTuning(int(inverse_column_tile), int(forward_column_tile), int(row_lanes))
+    st = lib.shhd_set_tuning(ctypes.byref(c), ctypes.byref(t))
+    if st != 0:
+        raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())
