@@ -90,10 +90,28 @@ def initial_state(cfg, seed=2):
     return shhd.hermitian_project(st, cfg)
 
 
-def time_gpu(cfg, steps, warmup):
-    st = initial_state(cfg)
+def device_initial_conditions(name):
+    """The `set` commands of the configurations that name reference styles,
+    evaluated on the device: C1 equilibrium/gaussian density and momentum
+    (seed 1, kBT 1e-4, rho0 1, cT 1); C3 the k^4 exp(-2 (k/8)^2) spectrum,
+    urms 1, seed 3, rho0 1.  None: the synthetic state of initial_state()."""
+    IC = shhd.initial_condition
+    if name == "ns_128":
+        return [IC(shhd.IC_EQUILIBRIUM_DENSITY, seed=1, kBT=1e-4, value=1.0, pressure_coefficient=1.0)] + \
+               [IC(shhd.IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE, component=d, seed=1, kBT=1e-4) for d in (0, 1)]
+    if name == "incompressible_4096":
+        return [IC(shhd.IC_EQUILIBRIUM_DENSITY, seed=3, kBT=0.0, value=1.0, pressure_coefficient=1.0)] + \
+               [IC(shhd.IC_SPECTRUM_K4, component=d, seed=3, n0=8.0, urms=1.0) for d in (0, 1)]
+    return None
+
+
+def time_gpu(cfg, steps, warmup, ics=None):
     with shhd.GpuFluidSolver(cfg) as g:
-        g.set_state(st)
+        if ics:
+            for ic in ics:
+                g.apply_initial_condition(ic)
+        else:
+            g.set_state(initial_state(cfg))
         g.step(warmup, 0.0)
         g.set_kernel_timing(True)
         t0 = time.perf_counter()
@@ -114,17 +132,25 @@ def ensemble_bench(ninst):
     launch per kernel, stage and launch shape, hipGraph replay) against the
     same runs done one after another with single handles."""
     import serve
-    jobs = serve.ensemble(ninst, nsteps=100)
+    import measures
+    jobs = serve.ensemble(ninst, nsteps=100, low_modes=16)
     srv = serve.JobServer(max_chunk=100, keep_results=False)
     try:
         rep = srv.run(jobs)
     finally:
         srv.close()
+    # time correlation of the 16 lowest modes of every realization, from its FP64 probe rows
+    c = np.array([measures.time_correlation(j.probe_rows, 10) for j in jobs])   # (inst, lag, mode)
+    decay = float(np.mean(c[:, 10].real / c[:, 0].real))
+    for j in jobs:
+        j.probes = None
     seq_s, _ = serve.run_sequential(jobs)
     pts = rep.point_steps
     return {"instances": ninst, "steps": 100, "Mpoint_steps_s": pts / rep.makespan / 1e6,
             "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.makespan,
-            "batched_s": rep.makespan, "sequential_s": seq_s}
+            "batched_s": rep.makespan, "sequential_s": seq_s,
+            "low_mode_correlation": {"modes": 16, "rows_per_instance": int(jobs[0].probe_rows.shape[0]),
+                                     "mean_C10_over_C0": decay}}
 
 
 def served_bench(njobs):
@@ -219,7 +245,8 @@ def main():
         tuning = {"inverse_column_tile": c1, "forward_column_tile": c3, "row_lanes": rnt, "given": True}
     else:
         tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
-    el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup)
+    ics = device_initial_conditions(args.workload)
+    el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup, ics)
     pts = cfg.nx_active * cfg.ny_active
     value = pts * args.steps / el / 1e6
 
@@ -252,7 +279,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 64 else "f32",
-        "data": "synthetic (rho=1+1e-3 noise, psi ~ 1e-2 Gaussian, small momentum; seeded)",
+        "data": ("device initial conditions of the configuration's `set` commands (seeded)" if ics else
+                 "synthetic (rho=1+1e-3 noise, psi ~ 1e-2 Gaussian, small momentum; seeded)"),
         "config": {"workload": desc, "grid": [cfg.nx_active, cfg.ny_active], "padded": [cfg.nx, cfg.ny],
                    "order_parameters": cfg.num_order_parameters, "integrator": "srk3",
                    "F_in": info.num_inverse_fields, "F_out": info.num_forward_fields,
@@ -266,7 +294,7 @@ def main():
     if args.precision == 64 and not args.no_fp32:
         cfg32, _ = workload(args.workload, 32)
         tuning32 = shhd.autotune(cfg32, steps=4) if not args.no_tune else None
-        el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup)
+        el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup, ics)
         v32 = pts * args.steps / el32 / 1e6
         result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value,
                           "launch_shapes": tuning32}

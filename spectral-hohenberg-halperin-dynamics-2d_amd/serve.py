@@ -30,12 +30,14 @@ class Job:
     cfg: shhd.SolverConfig
     make_state: Callable[[], np.ndarray]   # initial state, built when the job is admitted
     nsteps: int
+    probes: Optional[list] = None          # (source, index, gx, gy) recorded every step in FP64
     # filled by the server
     slot: int = -1
     remaining: int = 0
     admitted: float = -1.0
     finished: float = -1.0
     result: Optional[np.ndarray] = None
+    probe_rows: Optional[np.ndarray] = None
 
     @property
     def points(self) -> int:
@@ -99,6 +101,8 @@ class JobServer:
                     if e.status == shhd.ERR_FULL and resident:
                         break          # wait for memory / slots to free up
                     raise
+                if j.probes:
+                    self.batch.set_probes(j.slot, j.probes, capacity=j.nsteps)
                 j.remaining = j.nsteps
                 j.admitted = self.clock() - t0
                 resident.append(j)
@@ -121,6 +125,8 @@ class JobServer:
                 if j.remaining == 0:
                     if self.keep_results:
                         j.result = self.batch.get_state(j.slot)
+                    if j.probes:
+                        j.probe_rows = self.batch.read_probes(j.slot)
                     self.batch.retire(j.slot)
                     j.finished = self.clock() - t0
                     done.append(j)
@@ -218,14 +224,21 @@ def served_stream(njobs: int, rate: float, steps_range=(10, 200), seed: int = 5,
     return jobs
 
 
-def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512), seed: int = 7) -> List[Job]:
-    """Independent quiescent realizations of uneven sizes, all present at t = 0."""
+def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512), seed: int = 7,
+             low_modes: int = 0) -> List[Job]:
+    """Independent quiescent realizations of uneven sizes, all present at t = 0;
+    with low_modes > 0 each records its lowest Fourier modes of psi every step
+    (FP64 probe rows, for the time correlation)."""
+    import measures
     rng = np.random.default_rng(seed)
     jobs = []
     for i in range(ninst):
         na = int(rng.choice(sizes))
         cfg = quiescent_config(na, seed=seed0 + i)
         sseed = int(rng.integers(1 << 31))
-        jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps,
+        probes = None
+        if low_modes:
+            probes = [(shhd.PROBE_STATE, 1, gx, gy) for gx, gy in measures.lowest_modes(cfg, low_modes)]
+        jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps, probes=probes,
                         make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed), 0.1)))
     return jobs
