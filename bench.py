@@ -105,6 +105,27 @@ def device_initial_conditions(name):
     return None
 
 
+def fp32_error(cfg64, ics, steps=20):
+    """Relative L2 error of the FP32 form against FP64 after `steps`
+    deterministic steps (noise off: the two forms draw different Gaussians)
+    from the same initial state, over all fields."""
+    import dataclasses
+    out = {}
+    for prec in (64, 32):
+        c = dataclasses.replace(cfg64, precision=prec, noise_momentum=False, noise_order_parameter=False)
+        with shhd.GpuFluidSolver(c) as g:
+            if ics:
+                for ic in ics:
+                    g.apply_initial_condition(ic)
+            else:
+                g.set_state(initial_state(c))
+            g.step(steps)
+            out[prec] = g.get_state()
+    d = out[32] - out[64]
+    per_field = [float(np.linalg.norm(d[f]) / max(np.linalg.norm(out[64][f]), 1e-300)) for f in range(d.shape[0])]
+    return {"steps": steps, "rel_l2": float(np.linalg.norm(d) / np.linalg.norm(out[64])), "per_field": per_field}
+
+
 def time_gpu(cfg, steps, warmup, ics=None):
     with shhd.GpuFluidSolver(cfg) as g:
         if ics:
@@ -216,8 +237,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
-    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT",
-                    help="use these launch shapes (inverse tile, forward tile, row lanes) instead of tuning")
+    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT[,CT,RT]",
+                    help="use these launch shapes (inverse tile, forward tile, row lanes, column and row "
+                         "transform choice) instead of tuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
@@ -240,9 +262,11 @@ def main():
     cfg, desc = workload(args.workload, args.precision)
     # launch shapes for this problem shape, measured before the timed region
     if args.launch_shapes:
-        c1, c3, rnt = (int(v) for v in args.launch_shapes.split(","))
-        shhd.set_launch_shapes(cfg, c1, c3, rnt)
-        tuning = {"inverse_column_tile": c1, "forward_column_tile": c3, "row_lanes": rnt, "given": True}
+        vals = [int(v) for v in args.launch_shapes.split(",")] + [0, 0]
+        c1, c3, rnt, ct, rt = vals[:5]
+        shhd.set_launch_shapes(cfg, c1, c3, rnt, ct, rt)
+        tuning = {"inverse_column_tile": c1, "forward_column_tile": c3, "row_lanes": rnt, "column_transform": ct,
+                  "row_transform": rt, "given": True}
     else:
         tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
     ics = device_initial_conditions(args.workload)
@@ -297,7 +321,7 @@ def main():
         el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup, ics)
         v32 = pts * args.steps / el32 / 1e6
         result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value,
-                          "launch_shapes": tuning32}
+                          "launch_shapes": tuning32, "error_vs_fp64": fp32_error(cfg, ics)}
     if not args.no_serve:
         result["ensemble"] = ensemble_bench(args.ensemble)
         result["served"] = served_bench(args.serve_jobs)

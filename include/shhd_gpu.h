@@ -269,8 +269,9 @@ shhd_status shhd_gpu_set_kernel_timing(shhd_gpu* h, int enable);
 shhd_status shhd_gpu_get_kernel_timing(shhd_gpu* h, double* total_ms, long long* launches, int nclasses);
 
 /* ---- launch-shape autotuner (no reference counterpart) ------------------------
- * Times the candidate column tiles of the inverse and forward column passes
- * and the candidate lane counts of the row pass for this configuration's
+ * Times the candidate column tiles of the inverse and forward column passes,
+ * the candidate lane counts of the row pass and the transform builds / radix
+ * orders of the column and row FFTs for this configuration's
  * problem shape on a scratch instance (`steps` timed steps per candidate),
  * and records the fastest of each for the process: every handle or batch
  * instance of that shape created afterwards uses them.  Results are the same
@@ -279,6 +280,10 @@ typedef struct shhd_tuning {
     int inverse_column_tile;   /* kx columns per inverse-column workgroup     */
     int forward_column_tile;   /* kx columns per forward-column workgroup     */
     int row_lanes;             /* lanes per row-pass workgroup                */
+    int column_transform;      /* column FFTs: 0 fixed-length build where one
+                                  exists, 1 runtime plan radix 8 first,
+                                  2 runtime plan radix 4                      */
+    int row_transform;         /* row FFTs, the same choices                  */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

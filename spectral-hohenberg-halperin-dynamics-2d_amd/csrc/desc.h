@@ -117,6 +117,7 @@ struct Desc {
     uint32_t seed_lo, seed_hi;
 
     FftPlanDev plan_x, plan_y;
+    int fixed_y;                // column passes: fixed-length transform where one is built (else plan_y)
 };
 
 // Field pointers for one stage (static fields share storage across buffers).

@@ -27,6 +27,16 @@
 
 namespace shhd {
 
+// Column transforms of K1 and K3: the fixed-length build when the launch
+// tuning chose it (and one exists for this length and shape), else the
+// runtime plan with the tuned radix order.
+template <typename T, int NT, int VPT>
+__device__ __forceinline__ void col_fft(cx<T>* buf, int L, int nseq, const Desc& d, const cx<T>* __restrict__ tw,
+                                        int sign) {
+    if (d.fixed_y) lds_fft_any<T, NT, VPT>(buf, L, nseq, d.plan_y, tw, sign);
+    else lds_fft<T, NT, VPT>(buf, L, nseq, d.plan_y, tw, sign);
+}
+
 // XCD-aware workgroup order.  The dispatcher hands workgroup b to XCD b % 8,
 // and each XCD has its own L2; adjacent column tiles read and write
 // neighbouring segments of the same cache lines of the row-major T and U
@@ -78,7 +88,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         lds[lp(idx)] = v;
     }
     __syncthreads();
-    lds_fft_any<T, NT, VPT>(lds, L, C, d.plan_y, twy, +1);
+    col_fft<T, NT, VPT>(lds, L, C, d, twy, +1);
     cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
     for (int idx = threadIdx.x; idx < ncol; idx += NT) {
         const int y = idx / C;
@@ -479,7 +489,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             lds[lp(c * L + y)] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
         }
         __syncthreads();
-        lds_fft_any<T, NT, VPT>(lds, L, C, d.plan_y, twy, -1);
+        col_fft<T, NT, VPT>(lds, L, C, d, twy, -1);
 #pragma unroll
         for (int i = 0; i < ACC; ++i) {
             const int m = tid + NT * i;

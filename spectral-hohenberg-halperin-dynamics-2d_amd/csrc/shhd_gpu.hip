@@ -66,11 +66,13 @@ bool is_device_length(int n) {
     return n == 1;
 }
 
-FftPlanDev make_fft_plan(int L) {
+// Stockham plan of length L: radix 8 passes first (max_radix 8) or radix 4
+// (max_radix 4), then 2, then 3.
+FftPlanDev make_fft_plan(int L, int max_radix = 8) {
     FftPlanDev p{};
     std::vector<int> rad;
     int m = L;
-    while (m % 8 == 0 && m > 0) { rad.push_back(8); m /= 8; }
+    while (max_radix >= 8 && m % 8 == 0 && m > 0) { rad.push_back(8); m /= 8; }
     while (m % 4 == 0) { rad.push_back(4); m /= 4; }
     while (m % 2 == 0) { rad.push_back(2); m /= 2; }
     while (m % 3 == 0) { rad.push_back(3); m /= 3; }   // radix 3 last: every ns stays a power of two
@@ -709,6 +711,9 @@ shhd_status validate(const shhd_config& c) {
 // ---------------------------------------------------------------------------
 struct Tuning {
     int c1 = 0, c3 = 0, rnt = 0;   // 0: the untuned default
+    // transforms: 0 fixed-length build where one exists (default), 1 runtime
+    // plan with radix-8 passes first, 2 runtime plan with radix-4 passes
+    int col_transform = 0, row_transform = 0;
 };
 std::mutex g_tune_mutex;
 std::map<std::string, Tuning> g_tuned;
@@ -750,6 +755,10 @@ void apply_plans(Instance& h, const Tuning& t) {
     h.cols = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 32, t.c1);
     h.cols3 = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 16, t.c3);
     h.rows = plan_rows(d.mx, d.my, h.elem, std::max(nt_min, t.rnt));
+    if (t.row_transform != 0) h.rows.fl = 0;
+    h.desc.fixed_y = t.col_transform == 0 ? 1 : 0;
+    h.desc.plan_y = make_fft_plan(d.my, t.col_transform == 2 ? 4 : 8);
+    h.desc.plan_x = make_fft_plan(d.mx, t.row_transform == 2 ? 4 : 8);
     int npsi_roles = 0;
     for (int i = 0; i < d.n_in; ++i)
         if (d.in_role[i] >= ROLE_PSI0) npsi_roles = std::max(npsi_roles, d.in_role[i] - ROLE_PSI0 + 1);
@@ -1935,13 +1944,30 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
         if (ms[2] < bk) { bk = ms[2]; best.c3 = c; best_ms[2] = ms[2]; }
     }
     cur.c3 = best.c3;
-    // row-pass lanes
+    // column transforms (shared by both column passes: their summed time)
+    bk = 1e30;
+    for (int ct : {0, 1, 2}) {
+        Tuning t = cur;
+        t.col_transform = ct;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[0] + ms[2] < bk) { bk = ms[0] + ms[2]; best.col_transform = ct; best_ms[0] = ms[0]; best_ms[2] = ms[2]; }
+    }
+    cur.col_transform = best.col_transform;
+    // row-pass lanes, then row transforms
     bk = 1e30;
     for (int nt : row_nts) {
         Tuning t = cur;
         t.rnt = nt;
         if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
         if (ms[1] < bk) { bk = ms[1]; best.rnt = nt; best_ms[1] = ms[1]; }
+    }
+    cur.rnt = best.rnt;
+    bk = 1e30;
+    for (int rt : {0, 1, 2}) {
+        Tuning t = cur;
+        t.row_transform = rt;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[1] < bk) { bk = ms[1]; best.row_transform = rt; best_ms[1] = ms[1]; }
     }
     store_tuning(h.tune_key, best);
     return SHHD_OK;
@@ -2022,6 +2048,8 @@ shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out
     out->inverse_column_tile = best.c1;
     out->forward_column_tile = best.c3;
     out->row_lanes = best.rnt;
+    out->column_transform = best.col_transform;
+    out->row_transform = best.row_transform;
     for (int k = 0; k < kClasses; ++k) out->ms[k] = ms[k];
     return SHHD_OK;
 }
@@ -2033,9 +2061,11 @@ shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* t) {
     shhd_status st = prepare(*config, false, h, &none);
     if (st != SHHD_OK) return st;
     if (t->inverse_column_tile < 0 || t->forward_column_tile < 0 ||
-        (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024))
+        (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024) ||
+        t->column_transform < 0 || t->column_transform > 2 || t->row_transform < 0 || t->row_transform > 2)
         return set_err(nullptr, SHHD_ERR_INVALID, "invalid launch shapes");
-    store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes});
+    store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes, t->column_transform,
+                                     t->row_transform});
     return SHHD_OK;
 }
 
