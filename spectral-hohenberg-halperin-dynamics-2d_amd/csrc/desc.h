@@ -70,6 +70,7 @@ struct Desc {
     // spectral column of gx holds Nay slots c with ky = c (c < Nay/2) or
     // ky = c - Nay (c >= Nay/2); slot Nay/2 (ky = -Nay/2) is always zero.
     int nax, nay, mx, my, hxa;
+    int lnay, lhxa;             // log2 of nay, hxa when powers of two, else -1
     double lx, ly;
     double dkx, dky;            // 2 pi / Lx, 2 pi / Ly
     int N, nfields, mode;

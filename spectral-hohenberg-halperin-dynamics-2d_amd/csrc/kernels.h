@@ -48,6 +48,11 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// v / n for n > 0: a shift when ln = log2(n) >= 0 (n a power of two), else a division
+__device__ __forceinline__ int divn(int v, int n, int ln) { return ln >= 0 ? (v >> ln) : v / n; }
+// log2 of a power of two
+__device__ __forceinline__ int log2p2(int n) { return 31 - __clz(n); }
+
 // signed ky index of padded row gy (src/domain.cc:145-149)
 __device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2) ? gy : gy - my; }
 
@@ -75,24 +80,25 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
     const int ncol = C * L;
     const int half = d.nay / 2;
     const T scale = T(d.in_scale);
-    for (int idx = threadIdx.x; idx < ncol; idx += NT) {
-        const int c = idx / L;
-        const int y = idx - c * L;
+    for (int c = 0; c < C; ++c) {
         const int gx = gx0 + c;
-        cx<T> v = czero<T>();
-        const int ky = signed_gy(y, L);
-        if (gx < d.hxa && ky < half && -ky < half) {
-            const int cidx = ky >= 0 ? ky : ky + d.nay;
-            v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+        for (int y = threadIdx.x; y < L; y += NT) {
+            cx<T> v = czero<T>();
+            const int ky = signed_gy(y, L);
+            if (gx < d.hxa && ky < half && -ky < half) {
+                const int cidx = ky >= 0 ? ky : ky + d.nay;
+                v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+            }
+            lds[lp(c * L + y)] = v;
         }
-        lds[lp(idx)] = v;
     }
     __syncthreads();
     col_fft<T, NT, VPT>(lds, L, C, d, twy, +1);
     cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
+    const int lc = log2p2(C);
     for (int idx = threadIdx.x; idx < ncol; idx += NT) {
-        const int y = idx / C;
-        const int c = idx - y * C;
+        const int y = idx >> lc;
+        const int c = idx - (y << lc);
         const int gx = gx0 + c;
         if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = lds[lp(c * L + y)];
     }
@@ -182,8 +188,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
     cx<T>* __restrict__ ubuf = I.ubuf;
     const cx<T>* __restrict__ twx = I.twx;
     int* __restrict__ flag = I.flag;
-    const int R = I.R;
-    const int L = d.mx;
+    // a fixed-length build knows its row length and rows per workgroup
+    const int R = FL > 0 ? (NT * PPT) / FL : I.R;
+    const int L = FL > 0 ? FL : d.mx;
     const int my = d.my;
     const int hxa = d.hxa;
     const int y0 = local * R;
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
         const int nk = R * hxa;
         for (int idx = tid; idx < nk; idx += NT) {
-            const int r = idx / hxa;
+            const int r = divn(idx, hxa, d.lhxa);
             const int k = idx - r * hxa;
             const int y = y0 + r;
             if (y < my) {
@@ -459,7 +466,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         }
         if (sp.flux_first) {   // FluxBuffer::begin_step
             for (int m = tid; m < nmodes; m += NT) {
-                const int c = m / nay;
+                const int c = divn(m, nay, d.lnay);
                 const int gx = gx0 + c;
                 if (gx >= hxa) continue;
                 const size_t at = static_cast<size_t>(gx) * nay + (m - c * nay);
@@ -482,9 +489,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         const int u = g.contrib[ci];
         const ContribCoef& cf = g.coef[ci];
         const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
+        const int lc = log2p2(C);
         for (int idx = tid; idx < ncol; idx += NT) {
-            const int y = idx / C;
-            const int c = idx - y * C;
+            const int y = idx >> lc;
+            const int c = idx - (y << lc);
             const int gx = gx0 + c;
             lds[lp(c * L + y)] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
         }
@@ -494,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         for (int i = 0; i < ACC; ++i) {
             const int m = tid + NT * i;
             if (m >= nmodes) continue;
-            const int c = m / nay;
+            const int c = divn(m, nay, d.lnay);
             const int cidx = m - c * nay;
             const int gx = gx0 + c;
             if (gx >= hxa || cidx == half) continue;
@@ -552,7 +560,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     constexpr bool kFp32 = sizeof(T) == 4;
 
     for (int m = tid; m < nmodes; m += NT) {
-        const int c = m / nay;
+        const int c = divn(m, nay, d.lnay);
         const int cidx = m - c * nay;
         const int gx = gx0 + c;
         if (gx >= hxa) continue;

@@ -112,6 +112,7 @@ ColPlan plan_columns(int my, int nay, int hxa, size_t elem, int nt_min, int seg_
     int target = c_tile > 0 ? c_tile : std::max(1, static_cast<int>(seg_bytes / static_cast<int>(elem)));
     if (nt_min > 0) target = std::max(target, (nt_min * p.V) / my);
     int C = std::min(std::max(target, 1), pow2ceil(hxa));
+    while (C & (C - 1)) C &= C - 1;   // a power of two (the column kernels split indices by shifts)
     auto lds3 = [&](int c) { return static_cast<size_t>(std::max(lp_size(c * my), 2 * c * nay)) * elem; };
     while (C > 1 && (lds3(C) > kLdsCap3 || C * my > 1024 * p.V)) C >>= 1;
     p.C = C;
@@ -134,6 +135,7 @@ RowPlan plan_rows(int mx, int my, size_t elem, int nt_min) {
     p.R = std::min(p.R, my);
     p.nblocks = (my + p.R - 1) / p.R;
     p.fl = row_fixed_length(mx, p.NT, p.V);
+    if (p.fl && p.R != (p.NT * p.V) / p.fl) p.fl = 0;   // the build assumes the natural rows per workgroup
     p.lds = static_cast<size_t>(lp_size(p.R * mx)) * elem;
     return p;
 }
@@ -429,6 +431,13 @@ shhd_status build_desc(Instance* h) {
     d.hxa = c.nx_active / 2;
     d.lx = c.lx;
     d.ly = c.ly;
+    auto log2_or = [](int n) {
+        int l = 0;
+        while ((1 << l) < n) ++l;
+        return (1 << l) == n ? l : -1;
+    };
+    d.lnay = log2_or(d.nay);
+    d.lhxa = log2_or(d.hxa);
     d.dkx = 2.0 * 3.14159265358979323846 / c.lx;
     d.dky = 2.0 * 3.14159265358979323846 / c.ly;
     d.N = N;
