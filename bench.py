@@ -147,7 +147,7 @@ def time_gpu(cfg, steps, warmup, ics=None):
 
 
 def ensemble_bench(ninst):
-    """Batched ensemble (configs[3] shape): ninst quiescent phi4 realizations
+    """Batched ensemble (the quiescent-ensemble workload): ninst quiescent phi4 realizations
     with sizes drawn from {64,128,256,512}^2, 100 SRK3 steps each (the stated
     10^4 scaled down), dx = 1.5 for stability; all advanced together (one
     launch per kernel, stage and launch shape, hipGraph replay) against the
@@ -175,7 +175,7 @@ def ensemble_bench(ninst):
 
 
 def served_bench(njobs):
-    """Served stream (configs[4] shape, scaled down): njobs with Poisson
+    """Served stream (the mixed served-job workload, scaled down): njobs with Poisson
     arrivals at 20 jobs/s, grids {256,512,1024,2048}^2 (weights .4/.3/.2/.1),
     N in {0,1,2,4}, FP32 or FP64, steps log-uniform in [10, 200] (the stated
     [1000, 20000] / 100); continuous batching with admission by footprint."""
@@ -256,7 +256,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration of this path fits one MI355X and runs are "
-              "single-GPU only (see the sharding design in DESIGN.md).", file=sys.stderr)
+              "single-GPU only; multi-GPU runs would shard by instance.", file=sys.stderr)
         sys.exit(2)
 
     cfg, desc = workload(args.workload, args.precision)

@@ -312,8 +312,9 @@ typedef struct shhd_batch shhd_batch;
 shhd_status shhd_batch_create(size_t device_budget_bytes, int use_graphs, shhd_batch** out);
 /* device bytes an instance of this configuration would hold */
 shhd_status shhd_batch_footprint(const shhd_config* config, size_t* bytes);
-/* admit one simulation with its initial reference-layout state; SHHD_ERR_FULL
- * when the budget or the slots are used up */
+/* admit one simulation with its initial reference-layout state (NULL: every
+ * field zero, as a new reference State, for shhd_batch_apply_initial_condition);
+ * SHHD_ERR_FULL when the budget or the slots are used up */
 shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const double* state, size_t n, int* slot);
 /* advance every admitted instance by nsteps */
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps);

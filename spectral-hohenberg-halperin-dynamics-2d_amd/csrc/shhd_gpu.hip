@@ -2268,8 +2268,8 @@ shhd_status shhd_batch_footprint(const shhd_config* config, size_t* bytes) {
 }
 
 shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const double* state, size_t n, int* slot) {
-    if (!b || !config || !state || !slot) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
-    return b->eng.admit(*config, state, n, slot);
+    if (!b || !config || !slot) return set_err(b ? &b->eng.err : nullptr, SHHD_ERR_INVALID, "null argument");
+    return b->eng.admit(*config, state, n, slot);   // state NULL: all fields zero
 }
 
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps) {

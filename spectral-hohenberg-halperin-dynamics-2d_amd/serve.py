@@ -28,8 +28,9 @@ class Job:
     job_id: int
     arrival: float                 # seconds after the start of the stream
     cfg: shhd.SolverConfig
-    make_state: Callable[[], np.ndarray]   # initial state, built when the job is admitted
+    make_state: Optional[Callable[[], np.ndarray]]   # initial state built on the host at admission, or
     nsteps: int
+    ics: Optional[list] = None             # ... initial conditions evaluated on the device (make_state None)
     probes: Optional[list] = None          # (source, index, gx, gy) recorded every step in FP64
     # filled by the server
     slot: int = -1
@@ -96,11 +97,13 @@ class JobServer:
             while qi < len(pending) and pending[qi].arrival <= now:
                 j = pending[qi]
                 try:
-                    j.slot = self.batch.admit(j.cfg, j.make_state())
+                    j.slot = self.batch.admit(j.cfg, j.make_state() if j.make_state else None)
                 except shhd.ShhdError as e:
                     if e.status == shhd.ERR_FULL and resident:
                         break          # wait for memory / slots to free up
                     raise
+                for ic in j.ics or []:
+                    self.batch.apply_initial_condition(j.slot, ic)
                 if j.probes:
                     self.batch.set_probes(j.slot, j.probes, capacity=j.nsteps)
                 j.remaining = j.nsteps
@@ -143,7 +146,10 @@ def run_sequential(jobs: List[Job], clock: Callable[[], float] = time.perf_count
     out = []
     for j in sorted(jobs, key=lambda j: (j.arrival, j.job_id)):
         with shhd.GpuFluidSolver(j.cfg) as g:
-            g.set_state(j.make_state())
+            if j.make_state:
+                g.set_state(j.make_state())
+            for ic in j.ics or []:
+                g.apply_initial_condition(ic)
             g.step(j.nsteps)
             out.append(g.get_state())
     return clock() - t0, out
@@ -219,9 +225,21 @@ def served_stream(njobs: int, rate: float, steps_range=(10, 200), seed: int = 5,
         nsteps = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
         cfg = model_config(na, n, prec, seed=1000 + i)
         sseed = int(rng.integers(1 << 31))
-        jobs.append(Job(job_id=i, arrival=t, cfg=cfg, nsteps=nsteps,
-                        make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed))))
+        jobs.append(Job(job_id=i, arrival=t, cfg=cfg, nsteps=nsteps, make_state=None, ics=job_ics(cfg, sseed)))
     return jobs
+
+
+def job_ics(cfg: shhd.SolverConfig, seed: int):
+    """Initial state of a served job, evaluated on the device: uniform density
+    1, psi_a from `set order_parameter a equilibrium/gaussian kBT 1e-4 k0 1`
+    and momentum from `set momentum all equilibrium/gaussian/compressible
+    kBT 1e-4` (per-job seed)."""
+    IC = shhd.initial_condition
+    ics = [IC(shhd.IC_EQUILIBRIUM_DENSITY, seed=seed, kBT=0.0, value=1.0, pressure_coefficient=1.0)]
+    ics += [IC(shhd.IC_EQUILIBRIUM_ORDER_PARAMETER, component=a, seed=seed, kBT=1e-4, k0=1.0)
+            for a in range(cfg.num_order_parameters)]
+    ics += [IC(shhd.IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE, component=d, seed=seed, kBT=1e-4) for d in (0, 1)]
+    return ics
 
 
 def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512), seed: int = 7,

@@ -559,14 +559,19 @@ class Batch(_Measures):
             raise ShhdError(st, lib.shhd_batch_last_error(None).decode())
         return n.value
 
-    def admit(self, cfg: SolverConfig, state: np.ndarray) -> int:
+    def admit(self, cfg: SolverConfig, state: Optional[np.ndarray] = None) -> int:
+        """Admit a simulation; state None starts it at zero (then apply initial conditions)."""
         c = cfg.to_c()
-        s = np.ascontiguousarray(state, dtype=np.complex128)
-        if s.shape != cfg.state_shape():
-            raise ValueError(f"state shape {s.shape} != {cfg.state_shape()}")
-        flat = s.view(np.float64).reshape(-1)
         slot = ctypes.c_int()
-        self._check(self.lib.shhd_batch_admit(self.h, ctypes.byref(c), _dptr(flat), flat.size, ctypes.byref(slot)))
+        if state is None:
+            self._check(self.lib.shhd_batch_admit(self.h, ctypes.byref(c), None, 0, ctypes.byref(slot)))
+        else:
+            s = np.ascontiguousarray(state, dtype=np.complex128)
+            if s.shape != cfg.state_shape():
+                raise ValueError(f"state shape {s.shape} != {cfg.state_shape()}")
+            flat = s.view(np.float64).reshape(-1)
+            self._check(self.lib.shhd_batch_admit(self.h, ctypes.byref(c), _dptr(flat), flat.size,
+                                                  ctypes.byref(slot)))
         self.cfgs[slot.value] = cfg
         return slot.value
 

@@ -63,7 +63,7 @@ def check_case(cfg, state, nsteps, tol=FP64_TOL, t0=0.0):
 # compressible
 # ---------------------------------------------------------------------------
 def test_shear_wave_compressible_srk3():
-    """BASELINE C1 kT=0 variant at 16^2: exact shear-wave decay, z = -eta k^2 dt."""
+    """Compressible-NS kT=0 shear wave at 16^2: exact decay, z = -eta k^2 dt."""
     banner("shear wave compressible srk3 16^2 3/2")
     cfg = shhd.SolverConfig(nx_active=16, ny_active=16, dealias="3/2", lx=16.0, ly=16.0, dt=0.05,
                             pressure_coefficient=1.0, eta=0.1, zeta=0.1, momentum_advection=True)
@@ -76,7 +76,7 @@ def test_shear_wave_compressible_srk3():
 
 
 def test_compressible_ns_random_128():
-    """BASELINE C1 (compressible NS, N=0) at its own grid, 128^2 -> 192^2, kT=0."""
+    """Compressible NS, N=0, at 128^2 -> 192^2 (dx = 1), kT=0."""
     banner("compressible NS random 128^2 3/2 srk3 100 steps")
     cfg = shhd.SolverConfig(nx_active=128, ny_active=128, dealias="3/2", lx=128.0, ly=128.0, dt=0.05,
                             pressure_coefficient=1.0, eta=0.1, zeta=0.1, momentum_advection=True)
@@ -86,7 +86,7 @@ def test_compressible_ns_random_128():
 
 
 def test_model_h_n2_srk3():
-    """BASELINE C2 physics (Model-H, 2 phi4 fields) at 32^2 with a stable dt."""
+    """Model-H physics (2 phi4 fields) at 32^2 with a stable dt."""
     banner("model-H N=2 32^2 3/2 srk3 100 steps")
     cfg = shhd.SolverConfig(nx_active=32, ny_active=32, dealias="3/2", lx=32.0, ly=32.0, dt=0.005,
                             num_order_parameters=2, pressure_coefficient=1.0, eta=0.5, zeta=0.5,
@@ -156,7 +156,7 @@ def transverse(st, cfg):
 
 
 def test_incompressible_turbulence_srk3():
-    """BASELINE C3 physics at 64^2 (L = 2 pi), advection + viscosity."""
+    """Incompressible turbulence physics at 64^2 (L = 2 pi), advection + viscosity."""
     banner("incompressible turbulence 64^2 3/2 srk3 100 steps")
     cfg = shhd.SolverConfig(nx_active=64, ny_active=64, dealias="3/2", dt=1e-3, mode=shhd.MODE_INCOMPRESSIBLE,
                             eta=1e-2, pressure_coefficient=1.0, momentum_advection=True)
@@ -182,7 +182,7 @@ def test_incompressible_scalar_euler_gradient():
 # quiescent
 # ---------------------------------------------------------------------------
 def test_quiescent_phi4_srk3():
-    """BASELINE C4 physics (quiescent phi4) at 64^2 with dx = 1.5 (stable)."""
+    """Quiescent phi4 physics at 64^2 with dx = 1.5 (stable)."""
     banner("quiescent phi4 64^2 3/2 srk3 100 steps")
     cfg = shhd.SolverConfig(nx_active=64, ny_active=64, dealias="3/2", lx=96.0, ly=96.0, dt=0.02,
                             mode=shhd.MODE_QUIESCENT, num_order_parameters=1, mobility=[1.0],
@@ -211,7 +211,7 @@ def test_quiescent_euler_two_fields():
 @pytest.mark.parametrize("steps,tol", [(20, 1e-5)])
 def test_fp32_model_h(steps, tol):
     """FP32 form against the FP64 oracle: tolerance 1e-5 relative L2 per field
-    after 20 SRK3 steps (the north-star bound for the FP32 form)."""
+    after 20 SRK3 steps (the stated bound for the FP32 form)."""
     banner("fp32 model-H N=2 32^2")
     cfg = shhd.SolverConfig(nx_active=32, ny_active=32, dealias="3/2", lx=32.0, ly=32.0, dt=0.005,
                             num_order_parameters=2, pressure_coefficient=1.0, eta=0.5, zeta=0.5,

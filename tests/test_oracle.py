@@ -159,7 +159,7 @@ def test_incompressible_viscous_decay():
 
 
 def test_shear_wave_exact():
-    """BASELINE C1 kT = 0 variant: j_x = A sin(2 pi y/Ly), rho = 1 is an exact
+    """Compressible-NS kT = 0 workload: j_x = A sin(2 pi y/Ly), rho = 1 is an exact
     nonlinear solution decaying with z = -eta k^2 dt."""
     cfg = shhd.SolverConfig(nx_active=16, ny_active=16, dealias="3/2", lx=16.0, ly=16.0, dt=0.05,
                             pressure_coefficient=1.0, eta=0.1, zeta=0.1, momentum_advection=True)
