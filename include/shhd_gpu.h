@@ -124,6 +124,12 @@ typedef struct shhd_info {
     int stages_per_step;         /* 1 (Euler) or 3 (SRK3)                      */
     size_t state_doubles;        /* doubles in the reference-layout state      */
     size_t device_bytes;         /* device memory held by the handle           */
+    /* launch shapes in use: kx columns per workgroup of the inverse and
+     * forward column passes, lanes per row-pass workgroup, and each pass's
+     * transform build: 0 staged through LDS, 1 or 2 fused (first pass from
+     * HBM into registers, LDS exchanges over that many buffers) */
+    int inverse_column_tile, forward_column_tile, row_lanes;
+    int inverse_column_fused, forward_column_fused, row_fused;
 } shhd_info;
 
 typedef struct shhd_gpu shhd_gpu;
@@ -282,8 +288,14 @@ typedef struct shhd_tuning {
     int row_lanes;             /* lanes per row-pass workgroup                */
     int column_transform;      /* column FFTs: 0 fixed-length build where one
                                   exists, 1 runtime plan radix 8 first,
-                                  2 runtime plan radix 4                      */
+                                  2 runtime plan radix 4, 3 and 4 fused
+                                  fixed-length build with one and two LDS
+                                  buffers (where one is built for the length
+                                  and tile, else 0)                           */
     int row_transform;         /* row FFTs, the same choices                  */
+    int plane_block;           /* log2 of the kx columns per block of the
+                                  row-to-column planes (0: column-major; a
+                                  block that does not divide Nax/2 uses 0)    */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

@@ -71,6 +71,8 @@ struct Desc {
     // ky = c - Nay (c >= Nay/2); slot Nay/2 (ky = -Nay/2) is always zero.
     int nax, nay, mx, my, hxa;
     int lnay, lhxa;             // log2 of nay, hxa when powers of two, else -1
+    int ulg;                    // U planes in blocks of 2^ulg kx columns: element (kx, y)
+                                // at ((kx >> ulg) My + y) 2^ulg + (kx mod 2^ulg); 2^ulg divides Nax/2
     double lx, ly;
     double dkx, dky;            // 2 pi / Lx, 2 pi / Ly
     int N, nfields, mode;
@@ -163,8 +165,8 @@ struct InstDev {
     Desc d;
     const cx<T>* twx;         // twiddles exp(-2 pi i j / Mx), j < Mx
     const cx<T>* twy;         // same for My
-    cx<T>* tbuf;              // K1 -> K2 planes
-    cx<T>* ubuf;              // K2 -> K3 planes
+    cx<T>* tbuf;              // K1 -> K2 planes, row-major: [field][y][kx]
+    cx<T>* ubuf;              // K2 -> K3 planes, kx-blocked column-major (Desc::ulg)
     int* flag;                // zero-density flag
     const uint64_t* step;     // Philox step counter of this instance
     cx<T>* flux;              // step flux, compact slots [rho x, rho y, (psi x, psi y)*N,

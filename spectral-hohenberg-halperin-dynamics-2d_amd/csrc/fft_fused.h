@@ -1,0 +1,186 @@
+// fft_fused.h — fixed-length Stockham transforms whose first pass takes its
+// inputs from registers and whose last pass may leave its outputs in
+// registers, so the kernels load a transform's inputs from HBM straight into
+// the first butterflies and consume the last butterflies' outputs without an
+// LDS round trip.  The passes in between exchange through LDS, alternating
+// between two buffers (one barrier per pass) or reusing one (two barriers).
+//
+// Same Stockham pass as lds_fft.h (butterfly j of a sequence, k = j mod ns:
+// inputs x[j + q L/R] twiddled by w_L^{sign q k L/(ns R)}, outputs at
+// (j - k) R + k + p ns), with the sub-length ns no longer required to be a
+// power of two, so the radix-3 pass may come first.  Two radix orders:
+//   R3 last  (8s, 4, 2, then 3): the last pass's butterfly j (j < L/3) ends
+//            with the outputs j, j + L/3, j + 2L/3;
+//   R3 first (3, then 8s, 4, 2): the first pass's butterfly j takes the
+//            inputs j, j + L/3, j + 2L/3.
+// A transform with radix 3 last followed by one with radix 3 first therefore
+// hands each lane the same six points (two radix-3 butterflies of 3 points
+// when NT * 6 = nseq * L), which is how the row pass goes from the inverse to
+// the forward transform through registers.
+#pragma once
+
+#include "lds_fft.h"
+
+namespace shhd {
+
+constexpr int fz_radix(int L, bool r3_first, int P) {
+    if (r3_first && L % 3 == 0) return P == 0 ? 3 : ct_radix(L / 3, P - 1);
+    return ct_radix(L, P);
+}
+constexpr int fz_npass(int L, bool r3_first) {
+    int n = 0;
+    while (fz_radix(L, r3_first, n) != 0) ++n;
+    return n;
+}
+constexpr int fz_ns(int L, bool r3_first, int P) {
+    int ns = 1;
+    for (int p = 0; p < P; ++p) ns *= fz_radix(L, r3_first, p);
+    return ns;
+}
+
+// Butterflies of a radix-R pass over NSEQ sequences of L points on NT lanes:
+// lane t holds butterflies id = t + NT b, b < B; id = s M + j (sequence s,
+// butterfly j of M = L/R).
+template <int NT, int L, int NSEQ, int R>
+struct FzMap {
+    static constexpr int M = L / R;
+    static constexpr int NB = NSEQ * M;
+    static constexpr int B = (NB + NT - 1) / NT;
+    __device__ __forceinline__ static bool valid(int id) { return NB % NT == 0 || id < NB; }
+};
+
+// Two LDS buffers used alternately (NBUF = 2) or one buffer (NBUF = 1).
+// wr() is the buffer the next pass writes, rd() the one the last pass wrote.
+template <typename T, int NBUF>
+struct FzBufs {
+    cx<T>* b0;
+    cx<T>* b1;
+    int w;
+    __device__ __forceinline__ cx<T>* wr() const { return (NBUF == 2 && w) ? b1 : b0; }
+    __device__ __forceinline__ cx<T>* rd() const { return (NBUF == 2 && !w) ? b1 : b0; }
+    __device__ __forceinline__ void flip() { w ^= 1; }
+    // before a pass writes: with one buffer, every lane must be done reading it
+    __device__ __forceinline__ void before_write() const {
+        if constexpr (NBUF == 1) __syncthreads();
+    }
+};
+
+// Stockham outputs of the butterflies in v to LDS.
+template <typename T, int NT, int L, int NSEQ, int R, int NS>
+__device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R], int tid) {
+    using Mp = FzMap<NT, L, NSEQ, R>;
+#pragma unroll
+    for (int b = 0; b < Mp::B; ++b) {
+        const int id = tid + NT * b;
+        if (!Mp::valid(id)) continue;
+        const int s = id / Mp::M;
+        const int j = id - s * Mp::M;
+        const int k = j % NS;
+        const int o = s * L + (j - k) * R + k;
+        if constexpr (NS % 16 == 0) {
+            const int o0 = lp(o);
+#pragma unroll
+            for (int p = 0; p < R; ++p) out[o0 + p * (NS + NS / 16)] = v[b][p];
+        } else {
+#pragma unroll
+            for (int p = 0; p < R; ++p) out[lp(o + p * NS)] = v[b][p];
+        }
+    }
+}
+
+// Butterfly inputs from LDS, twiddled, transformed in registers.
+template <typename T, int NT, int L, int NSEQ, int R, int NS>
+__device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R],
+                                        const cx<T>* __restrict__ tw, int sign, int tid) {
+    using Mp = FzMap<NT, L, NSEQ, R>;
+    constexpr int TSTEP = L / (NS * R);
+#pragma unroll
+    for (int b = 0; b < Mp::B; ++b) {
+        const int id = tid + NT * b;
+        if (!Mp::valid(id)) continue;
+        const int s = id / Mp::M;
+        const int j = id - s * Mp::M;
+        const int a = s * L + j;
+        if constexpr (Mp::M % 16 == 0) {
+            const int a0 = lp(a);
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b][q] = in[a0 + q * (Mp::M + Mp::M / 16)];
+        } else {
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b][q] = in[lp(a + q * Mp::M)];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < Mp::B; ++b) {
+        const int id = tid + NT * b;
+        if (!Mp::valid(id)) continue;
+        if constexpr (NS > 1) {
+            const int j = id - (id / Mp::M) * Mp::M;
+            const int k = j % NS;
+            if (k != 0) apply_twiddles<T, R>(v[b], tw, k * TSTEP, sign);
+        }
+        dft_small<T, R>(v[b], sign);
+    }
+}
+
+// First pass (ns = 1, no twiddles) on inputs the caller placed in v:
+// v[b][q] = x_s[j + q L/R] for butterfly id = tid + NT b = s (L/R) + j.
+template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF>
+__device__ __forceinline__ void fz_first(FzBufs<T, NBUF>& bf,
+                                         cx<T> (&v)[FzMap<NT, L, NSEQ, fz_radix(L, R3F, 0)>::B][fz_radix(L, R3F, 0)],
+                                         int sign, int tid) {
+    constexpr int R = fz_radix(L, R3F, 0);
+    using Mp = FzMap<NT, L, NSEQ, R>;
+    // an opaque copy of the lane index: the constant-folded LDS offsets of
+    // each call are formed here, not hoisted out of the caller's loops
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int b = 0; b < Mp::B; ++b)
+        if (Mp::valid(tid + NT * b)) dft_small<T, R>(v[b], sign);
+    bf.before_write();
+    fz_write<T, NT, L, NSEQ, R, 1>(bf.wr(), v, tid);
+    bf.flip();
+}
+
+// Pass P (0 < P) from LDS to LDS.
+template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF, int P>
+__device__ __forceinline__ void fz_pass(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign, int tid) {
+    constexpr int R = fz_radix(L, R3F, P);
+    constexpr int NS = fz_ns(L, R3F, P);
+    cx<T> v[FzMap<NT, L, NSEQ, R>::B][R];
+    __syncthreads();
+    fz_read<T, NT, L, NSEQ, R, NS>(bf.rd(), v, tw, sign, tid);
+    bf.before_write();
+    fz_write<T, NT, L, NSEQ, R, NS>(bf.wr(), v, tid);
+    bf.flip();
+}
+
+template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF, int... P>
+__device__ __forceinline__ void fz_passes(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign, int tid,
+                                          std::integer_sequence<int, P...>) {
+    (fz_pass<T, NT, L, NSEQ, R3F, NBUF, P + 1>(bf, tw, sign, tid), ...);
+}
+
+// Passes 1 .. npass-1, every one through LDS: the transform ends in LDS
+// (bf.rd(), natural order, after a barrier the caller issues).
+template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF>
+__device__ __forceinline__ void fz_rest_to_lds(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign, int tid) {
+    asm volatile("" : "+v"(tid));
+    fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, fz_npass(L, R3F) - 1>{});
+}
+
+// Passes 1 .. npass-1 with the last one left in registers: v[b][p] = y_s[j + p L/R]
+// for butterfly id = tid + NT b = s (L/R) + j of the last (radix R) pass.
+template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF>
+__device__ __forceinline__ void fz_rest_to_regs(
+    FzBufs<T, NBUF>& bf,
+    cx<T> (&v)[FzMap<NT, L, NSEQ, fz_radix(L, R3F, fz_npass(L, R3F) - 1)>::B][fz_radix(L, R3F, fz_npass(L, R3F) - 1)],
+    const cx<T>* __restrict__ tw, int sign, int tid) {
+    constexpr int NP = fz_npass(L, R3F);
+    asm volatile("" : "+v"(tid));
+    fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 2>{});
+    __syncthreads();
+    fz_read<T, NT, L, NSEQ, fz_radix(L, R3F, NP - 1), fz_ns(L, R3F, NP - 1)>(bf.rd(), v, tw, sign, tid);
+}
+
+}  // namespace shhd

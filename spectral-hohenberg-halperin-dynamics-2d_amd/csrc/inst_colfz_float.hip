@@ -1,0 +1,7 @@
+// Explicit instances of the fused K1 and K3 builds for float.
+#define SHHD_DEFINE_INSTANCES
+#include "instances.h"
+namespace shhd {
+#define SHHD_B(nt, len) SHHD_I_COLFZ(template, float, nt, len)
+SHHD_FOR_COL_FZ(SHHD_B)
+}  // namespace shhd

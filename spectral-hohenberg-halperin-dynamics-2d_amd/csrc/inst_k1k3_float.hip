@@ -1,10 +1,8 @@
-// Explicit instances of K1 and K3 for float.
+// Explicit instances of the staged K1 and K3 and of K2 (any length, and the
+// physical-field row pass) for float.
 #define SHHD_DEFINE_INSTANCES
 #include "instances.h"
 namespace shhd {
-#define SHHD_INST(nt, v) \
-    template __global__ void k_inverse_columns<float, nt, v>(const InstDev<float>*, const StagePtrs<float>*, BlockMap); \
-    template __global__ void k_forward_columns<float, nt, v, 4>(const InstDev<float>*, const StagePtrs<float>*, StageParams, BlockMap); \
-    template __global__ void k_forward_columns<float, nt, v, v>(const InstDev<float>*, const StagePtrs<float>*, StageParams, BlockMap);
-SHHD_FOR_NT_V(SHHD_INST)
+#define SHHD_B(nt, v) SHHD_I_SHAPE(template, float, nt, v)
+SHHD_FOR_NT_V(SHHD_B)
 }  // namespace shhd

@@ -8,40 +8,60 @@
 // X(NT, V): NT lanes per workgroup, V values per lane (6 for 3*2^a sides, 8 for 2^a)
 #define SHHD_FOR_NT_V(X) X(64, 6) X(64, 8) X(256, 6) X(256, 8) X(512, 6) X(512, 8) X(1024, 6) X(1024, 8)
 
-namespace shhd {
-
-#define SHHD_DECL_K1(T, nt, v) \
-    extern template __global__ void k_inverse_columns<T, nt, v>(const InstDev<T>*, const StagePtrs<T>*, BlockMap);
-#define SHHD_DECL_K3(T, nt, v, a) \
-    extern template __global__ void k_forward_columns<T, nt, v, a>(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
-#define SHHD_DECL_K2(T, nt, v, mn) \
-    extern template __global__ void k_rows<T, nt, v, mn, 0>(const InstDev<T>*, BlockMap);
-#define SHHD_DECL_K2F(T, nt, v, mn, fl) \
-    extern template __global__ void k_rows<T, nt, v, mn, fl>(const InstDev<T>*, BlockMap);
-
 // Row kernels built for one row length (3/2-padded grids of 512^2 .. 4096^2):
 // X(NT, V, L), the lane shape the row plan picks for rows of L points.
 #define SHHD_FOR_ROW_FL(X) X(256, 6, 768) X(256, 6, 1536) X(512, 6, 3072) X(1024, 6, 6144)
-#define SHHD_DECL_K2F_ALL(T, nt, v, fl) \
-    SHHD_DECL_K2F(T, nt, v, 0, fl) SHHD_DECL_K2F(T, nt, v, 1, fl) SHHD_DECL_K2F(T, nt, v, 2, fl) \
-    SHHD_DECL_K2F(T, nt, v, 4, fl) SHHD_DECL_K2F(T, nt, v, 8, fl)
-#define SHHD_DECL_K2F_F64(nt, v, fl) SHHD_DECL_K2F_ALL(double, nt, v, fl)
-#define SHHD_DECL_K2F_F32(nt, v, fl) SHHD_DECL_K2F_ALL(float, nt, v, fl)
 
-#define SHHD_DECL_KP(T, nt, v) \
-    extern template __global__ void k_rows_physical<T, nt, v>(const InstDev<T>*, T*, int);
+// Column kernels with fused transforms: X(NT, L), C = NT * 6 / L columns of
+// L points per workgroup (3/2-padded sides 64 .. 4096).
+#define SHHD_FOR_COL_FZ(X)                                                                            \
+    X(256, 96) X(64, 192) X(256, 192) X(64, 384) X(256, 384) X(256, 768) X(512, 768) X(256, 1536)    \
+    X(512, 1536) X(512, 3072) X(1024, 3072) X(1024, 6144)
 
-#define SHHD_DECL_ALL(T, nt, v)                                                              \
-    SHHD_DECL_K1(T, nt, v) SHHD_DECL_K3(T, nt, v, 4) SHHD_DECL_K3(T, nt, v, v) SHHD_DECL_KP(T, nt, v) \
-    SHHD_DECL_K2(T, nt, v, 0) SHHD_DECL_K2(T, nt, v, 1) SHHD_DECL_K2(T, nt, v, 2)            \
-    SHHD_DECL_K2(T, nt, v, 4) SHHD_DECL_K2(T, nt, v, 8)
-#define SHHD_DECL_F64(nt, v) SHHD_DECL_ALL(double, nt, v)
-#define SHHD_DECL_F32(nt, v) SHHD_DECL_ALL(float, nt, v)
+// order-parameter counts the row kernel is built for (fields held per lane)
+#define SHHD_FOR_MAXN(X, ...) X(__VA_ARGS__, 0) X(__VA_ARGS__, 1) X(__VA_ARGS__, 2) X(__VA_ARGS__, 4) X(__VA_ARGS__, 8)
+
+namespace shhd {
+
+// K1<T, NT, V, FZL, NBUF>, K3<T, NT, V, ACC, FZL, NBUF>, K2<T, NT, V, MAXN, FL, NBUF>
+#define SHHD_K1_SIG(T) (const InstDev<T>*, const StagePtrs<T>*, BlockMap)
+#define SHHD_K3_SIG(T) (const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap)
+#define SHHD_K2_SIG(T) (const InstDev<T>*, BlockMap)
+#define SHHD_KP_SIG(T) (const InstDev<T>*, T*, int)
+
+// EXT is `extern template` for the declarations, `template` in the unit that builds them
+#define SHHD_I_K1(EXT, T, nt, v, fz, nb) EXT __global__ void k_inverse_columns<T, nt, v, fz, nb> SHHD_K1_SIG(T);
+#define SHHD_I_K3(EXT, T, nt, v, a, fz, nb) EXT __global__ void k_forward_columns<T, nt, v, a, fz, nb> SHHD_K3_SIG(T);
+#define SHHD_I_K2(EXT, T, nt, v, fl, nb, mn) EXT __global__ void k_rows<T, nt, v, mn, fl, nb> SHHD_K2_SIG(T);
+#define SHHD_I_KP(EXT, T, nt, v) EXT __global__ void k_rows_physical<T, nt, v> SHHD_KP_SIG(T);
+
+// staged column and row kernels of every lane shape, any length
+#define SHHD_I_SHAPE(EXT, T, nt, v)                                                              \
+    SHHD_I_K1(EXT, T, nt, v, 0, 1) SHHD_I_K3(EXT, T, nt, v, 4, 0, 1) SHHD_I_K3(EXT, T, nt, v, v, 0, 1) \
+    SHHD_I_KP(EXT, T, nt, v) SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, 0, 0)
+// fixed-length row kernels: staged, fused with one and with two LDS buffers
+#define SHHD_I_ROWFL(EXT, T, nt, v, fl)                                                              \
+    SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 0) SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 1)  \
+    SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 2)
+// fused column kernels, one and two LDS buffers
+#define SHHD_I_COLFZ(EXT, T, nt, len)                                                                \
+    SHHD_I_K1(EXT, T, nt, 6, len, 1) SHHD_I_K1(EXT, T, nt, 6, len, 2) SHHD_I_K3(EXT, T, nt, 6, 4, len, 1) \
+    SHHD_I_K3(EXT, T, nt, 6, 4, len, 2)
+
+#define SHHD_EXTERN extern template
+#define SHHD_D_SHAPE_F64(nt, v) SHHD_I_SHAPE(SHHD_EXTERN, double, nt, v)
+#define SHHD_D_SHAPE_F32(nt, v) SHHD_I_SHAPE(SHHD_EXTERN, float, nt, v)
+#define SHHD_D_ROWFL_F64(nt, v, fl) SHHD_I_ROWFL(SHHD_EXTERN, double, nt, v, fl)
+#define SHHD_D_ROWFL_F32(nt, v, fl) SHHD_I_ROWFL(SHHD_EXTERN, float, nt, v, fl)
+#define SHHD_D_COLFZ_F64(nt, len) SHHD_I_COLFZ(SHHD_EXTERN, double, nt, len)
+#define SHHD_D_COLFZ_F32(nt, len) SHHD_I_COLFZ(SHHD_EXTERN, float, nt, len)
 #ifndef SHHD_DEFINE_INSTANCES
-SHHD_FOR_NT_V(SHHD_DECL_F64)
-SHHD_FOR_NT_V(SHHD_DECL_F32)
-SHHD_FOR_ROW_FL(SHHD_DECL_K2F_F64)
-SHHD_FOR_ROW_FL(SHHD_DECL_K2F_F32)
+SHHD_FOR_NT_V(SHHD_D_SHAPE_F64)
+SHHD_FOR_NT_V(SHHD_D_SHAPE_F32)
+SHHD_FOR_ROW_FL(SHHD_D_ROWFL_F64)
+SHHD_FOR_ROW_FL(SHHD_D_ROWFL_F32)
+SHHD_FOR_COL_FZ(SHHD_D_COLFZ_F64)
+SHHD_FOR_COL_FZ(SHHD_D_COLFZ_F32)
 #endif
 
 }  // namespace shhd

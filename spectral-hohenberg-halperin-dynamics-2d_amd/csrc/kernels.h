@@ -24,6 +24,7 @@
 
 #include "../../include/shhd_gpu.h"
 #include "desc.h"
+#include "fft_fused.h"
 
 namespace shhd {
 
@@ -59,7 +60,11 @@ __device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2)
 // ---------------------------------------------------------------------------
 // K1
 // ---------------------------------------------------------------------------
-template <typename T, int NT, int VPT>
+// FZL > 0: a build for columns of exactly FZL points, C = NT * 6 / FZL per
+// workgroup, whose first transform pass loads the compact spectra straight
+// into its butterflies (no LDS staging) and whose passes alternate between
+// NBUF LDS buffers.
+template <typename T, int NT, int VPT, int FZL, int NBUF>
 __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __restrict__ insts,
                                                         const StagePtrs<T>* __restrict__ ptrs, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -80,42 +85,92 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
     const int ncol = C * L;
     const int half = d.nay / 2;
     const T scale = T(d.in_scale);
-    for (int c = 0; c < C; ++c) {
-        const int gx = gx0 + c;
-        for (int y = threadIdx.x; y < L; y += NT) {
-            cx<T> v = czero<T>();
-            const int ky = signed_gy(y, L);
-            if (gx < d.hxa && ky < half && -ky < half) {
-                const int cidx = ky >= 0 ? ky : ky + d.nay;
-                v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+    const cx<T>* res = lds;   // the transformed columns, natural order
+    if constexpr (FZL > 0) {
+        constexpr int CF = NT * 6 / FZL;
+        constexpr int R0 = fz_radix(FZL, false, 0);
+        using M0 = FzMap<NT, FZL, CF, R0>;
+        const int tid = threadIdx.x;
+        FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
+        cx<T> v[M0::B][R0];
+#pragma unroll
+        for (int b = 0; b < M0::B; ++b) {
+            const int id = tid + NT * b;
+            if (!M0::valid(id)) continue;
+            const int s = id / M0::M;
+            const int j = id - s * M0::M;
+            const int gx = gx0 + s;
+#pragma unroll
+            for (int q = 0; q < R0; ++q) {
+                const int ky = signed_gy(j + q * M0::M, FZL);
+                v[b][q] = czero<T>();
+                if (gx < d.hxa && ky < half && -ky < half) {
+                    const int cidx = ky >= 0 ? ky : ky + d.nay;
+                    v[b][q] = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+                }
             }
-            lds[lp(c * L + y)] = v;
         }
+        fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, +1, tid);
+        fz_rest_to_lds<T, NT, FZL, CF, false, NBUF>(bf, twy, +1, tid);
+        __syncthreads();
+        res = bf.rd();
+    } else {
+        for (int c = 0; c < C; ++c) {
+            const int gx = gx0 + c;
+            for (int y = threadIdx.x; y < L; y += NT) {
+                cx<T> v = czero<T>();
+                const int ky = signed_gy(y, L);
+                if (gx < d.hxa && ky < half && -ky < half) {
+                    const int cidx = ky >= 0 ? ky : ky + d.nay;
+                    v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+                }
+                lds[lp(c * L + y)] = v;
+            }
+        }
+        __syncthreads();
+        col_fft<T, NT, VPT>(lds, L, C, d, twy, +1);
     }
-    __syncthreads();
-    col_fft<T, NT, VPT>(lds, L, C, d, twy, +1);
     cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
     const int lc = log2p2(C);
     for (int idx = threadIdx.x; idx < ncol; idx += NT) {
         const int y = idx >> lc;
         const int c = idx - (y << lc);
         const int gx = gx0 + c;
-        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = lds[lp(c * L + y)];
+        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = res[lp(c * L + y)];
     }
 }
 
 // ---------------------------------------------------------------------------
 // K2
 // ---------------------------------------------------------------------------
+// Physical fields of a lane's points.  `rho` holds the density after the
+// inverse transforms and 1/rho (1/rho0 when incompressible) once the velocity
+// is made; v = j * (1/rho) is formed where a product needs it, which keeps two
+// arrays fewer in registers.
 template <typename T, int PPT, int MAXN>
 struct RowFields {
-    T rho[PPT], jx[PPT], jy[PPT], vx[PPT], vy[PPT];
+    T rho[PPT], jx[PPT], jy[PPT];
     T psi[MAXN > 0 ? MAXN : 1][PPT];
+
+    __device__ __forceinline__ T vx(int i) const { return jx[i] * rho[i]; }
+    __device__ __forceinline__ T vy(int i) const { return jy[i] * rho[i]; }
+    // opaque to the optimizer: the velocity products of each output pass are
+    // formed in that pass, not hoisted out of the pass loop and held live
+    __device__ __forceinline__ void fence() {
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) asm volatile("" : "+v"(rho[i]));
+    }
+    // rho -> 1/rho (compressible) or 1/rho0; true when the density is zero
+    __device__ __forceinline__ bool make_velocity(int i, bool compressible, T r0) {
+        const T r = compressible ? rho[i] : r0;
+        rho[i] = T(1) / r;
+        return compressible && r == T(0);
+    }
 
     __device__ __forceinline__ void clear() {
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
-            rho[i] = jx[i] = jy[i] = vx[i] = vy[i] = T(0);
+            rho[i] = jx[i] = jy[i] = T(0);
 #pragma unroll
             for (int a = 0; a < (MAXN > 0 ? MAXN : 1); ++a) psi[a][i] = T(0);
         }
@@ -144,21 +199,21 @@ struct RowFields {
                 const T p = psi_of(it.alpha, i);
                 return T(d.phi4_u[it.alpha]) * p * p * p;   // src/model_free_energy_phi4.cc:54
             }
-            case OUT_PSIVX: return psi_of(it.alpha, i) * vx[i];
-            case OUT_PSIVY: return psi_of(it.alpha, i) * vy[i];
+            case OUT_PSIVX: return psi_of(it.alpha, i) * vx(i);
+            case OUT_PSIVY: return psi_of(it.alpha, i) * vy(i);
             case OUT_GRADF: {
                 T s = T(0);
                 for (int g = 0; g < d.n_grad; ++g) {
                     if (d.grad_component[g] != it.alpha) continue;
-                    s += T(-d.grad_amplitude[g]) * (d.grad_direction[g] == 0 ? vx[i] : vy[i]);
+                    s += T(-d.grad_amplitude[g]) * (d.grad_direction[g] == 0 ? vx(i) : vy(i));
                 }
                 return s;
             }
-            case OUT_VX: return vx[i];
-            case OUT_VY: return vy[i];
-            case OUT_PXX: return jx[i] * vx[i];
-            case OUT_PXY: return jx[i] * vy[i];
-            default: return jy[i] * vy[i];
+            case OUT_VX: return vx(i);
+            case OUT_VY: return vy(i);
+            case OUT_PXX: return jx[i] * vx(i);
+            case OUT_PXY: return jx[i] * vy(i);
+            default: return jy[i] * vy(i);
         }
     }
 };
@@ -176,12 +231,22 @@ __device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPl
     else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
 }
 
-template <typename T, int NT, int PPT, int MAXN, int FL>
+// NBUF > 0 (FL > 0 and PPT = 6 only): the inverse transform runs with the
+// radix-3 pass last and leaves each lane the six points j, j + FL/3, j + 2FL/3
+// of its two radix-3 butterflies; the products at those points feed the
+// forward transform's radix-3 first pass straight from registers, the
+// inverse's first pass loads T straight into its butterflies, and the passes
+// alternate between NBUF LDS buffers.  NBUF = 0: every transform staged
+// through one LDS buffer.
+template <typename T, int NT, int PPT, int MAXN, int FL, int NBUF>
 __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const int inst = map.inst[blockIdx.x];
-    const int local = blockIdx.x - map.first[inst];
+    // contiguous row ranges per XCD: the column-major U lines that
+    // neighbouring rows fill meet in one L2
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int inst = map.inst[blk];
+    const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
     const Desc& d = I.d;
     const cx<T>* __restrict__ tbuf = I.tbuf;
@@ -200,6 +265,118 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 
     RowFields<T, PPT, MAXN> F;
     F.clear();
+
+    if constexpr (NBUF > 0) {
+        static_assert(FL > 0 && PPT == 6, "fused row transforms need a fixed length and six points per lane");
+        constexpr int RF = NT * 6 / FL;              // rows per workgroup
+        constexpr int M3 = FL / 3;
+        constexpr int R0 = fz_radix(FL, false, 0);
+        using MI = FzMap<NT, FL, RF, R0>;
+        static_assert(FzMap<NT, FL, RF, 3>::B == 2 && FzMap<NT, FL, RF, 3>::NB == 2 * NT, "two radix-3 butterflies per lane");
+        FzBufs<T, NBUF> bf{lds, lds + lp_size(RF * FL), 0};
+
+        // ---- inverse x transforms, two fields per complex FFT -------------
+        for (int s = 0; s < d.n_in; s += 2) {
+            const bool has_b = s + 1 < d.n_in;
+            const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
+            const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
+            cx<T> v[MI::B][R0];
+            // an opaque copy of the lane index: the load offsets are formed
+            // per transform, not held live across the loop
+            int lane = tid;
+            asm volatile("" : "+v"(lane));
+#pragma unroll
+            for (int b = 0; b < MI::B; ++b) {
+                const int id = lane + NT * b;
+                if (!MI::valid(id)) continue;
+                const int r = id / MI::M;
+                const int j = id - r * MI::M;
+                const int y = y0 + r;
+                const size_t row = static_cast<size_t>(y < my ? y : 0) * hxa;
+#pragma unroll
+                for (int q = 0; q < R0; ++q) {
+                    // one load per field: kx = k (k < Nax/2) or the mirror
+                    // kx = FL - k (k > FL - Nax/2) of the Hermitian row
+                    const int k = j + q * MI::M;
+                    const bool mirror = k > FL - hxa;
+                    const int kk = mirror ? FL - k : k;
+                    cx<T> z = czero<T>();
+                    if (y < my && (k < hxa || mirror)) {
+                        cx<T> a = ta[row + kk];
+                        cx<T> bb = has_b ? tb[row + kk] : czero<T>();
+                        if (k == 0) { a.im = T(0); bb.im = T(0); }
+                        if (mirror) { a.im = -a.im; bb.im = -bb.im; }   // conj(A) + i conj(B)
+                        z = mk(a.re - bb.im, a.im + bb.re);             // A + iB
+                    }
+                    v[b][q] = z;
+                }
+            }
+            fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
+            cx<T> w[2][3];
+            fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
+            const int ra = d.in_role[s];
+            const int rb = has_b ? d.in_role[s + 1] : -1;
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    F.store(ra, 3 * b + p, w[b][p].re);
+                    if (rb >= 0) F.store(rb, 3 * b + p, w[b][p].im);
+                }
+        }
+
+        // ---- velocity (src/fcalculator_workspace.cc:122-133) -----------------
+        if (d.make_velocity) {
+            const T r0 = T(d.rho0);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const int id = tid + NT * (i / 3);
+                if (F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + id / M3 < my) atomicOr(flag, 1);
+            }
+        }
+
+        // ---- products and forward x transforms, two per complex FFT ---------
+        for (int o = 0; o < d.n_out; o += 2) {
+            const bool has_b = o + 1 < d.n_out;
+            const OutItem ia = d.out[o];
+            const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
+            F.fence();
+            cx<T> v[2][3];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    const T va = F.eval(d, ia, 3 * b + p);
+                    const T vb = has_b ? F.eval(d, ib, 3 * b + p) : T(0);
+                    v[b][p] = mk(va, vb);
+                }
+            fz_first<T, NT, FL, RF, true, NBUF>(bf, v, -1, tid);
+            fz_rest_to_lds<T, NT, FL, RF, true, NBUF>(bf, twx, -1, tid);
+            __syncthreads();
+            const cx<T>* res = bf.rd();
+            cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
+            cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
+            // U planes in kx blocks of 2^ulg columns over all rows: the lanes
+            // of one block write one segment per row
+            const int nk = RF * hxa;
+            const int umask = (1 << d.ulg) - 1;
+            for (int idx = tid; idx < nk; idx += NT) {
+                const int kb = idx / (RF << d.ulg);
+                const int rem = idx - kb * (RF << d.ulg);
+                const int r = rem >> d.ulg;
+                const int k = (kb << d.ulg) + (rem & umask);
+                const int y = y0 + r;
+                if (y < my) {
+                    const cx<T> zk = res[lp(r * FL + k)];
+                    const cx<T> zm = res[lp(r * FL + (k == 0 ? 0 : FL - k))];
+                    const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);
+                    ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
+                    if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
+                }
+            }
+        }
+        return;
+    }
 
     // ---- inverse x transforms, two fields per complex FFT -----------------
     for (int s = 0; s < d.n_in; s += 2) {
@@ -249,17 +426,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const int idx = tid + NT * i;
-            if (idx < npts) {
-                if (d.velocity_compressible) {
-                    const T r = F.rho[i];
-                    if (r == T(0) && y0 + idx / L < my) atomicOr(flag, 1);
-                    F.vx[i] = F.jx[i] / r;
-                    F.vy[i] = F.jy[i] / r;
-                } else {
-                    F.vx[i] = F.jx[i] / r0;
-                    F.vy[i] = F.jy[i] / r0;
-                }
-            }
+            if (idx < npts && F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + idx / L < my)
+                atomicOr(flag, 1);
         }
     }
 
@@ -268,6 +436,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         const bool has_b = o + 1 < d.n_out;
         const OutItem ia = d.out[o];
         const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
+        F.fence();
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const int idx = tid + NT * i;
@@ -282,14 +451,17 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
         cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
         const int nk = R * hxa;
+        const int umask = (1 << d.ulg) - 1;
         for (int idx = tid; idx < nk; idx += NT) {
-            const int r = divn(idx, hxa, d.lhxa);
-            const int k = idx - r * hxa;
+            const int kb = idx / (R << d.ulg);
+            const int rem = idx - kb * (R << d.ulg);
+            const int r = rem >> d.ulg;
+            const int k = (kb << d.ulg) + (rem & umask);
             const int y = y0 + r;
             if (y < my) {
                 const cx<T> zk = lds[lp(r * L + k)];
                 const cx<T> zm = lds[lp(r * L + (k == 0 ? 0 : L - k))];
-                const size_t at = static_cast<size_t>(y) * hxa + k;
+                const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);   // kx-blocked U
                 ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
                 if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
             }
@@ -408,7 +580,14 @@ __device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<dou
 // ---------------------------------------------------------------------------
 // K3
 // ---------------------------------------------------------------------------
-template <typename T, int NT, int VPT, int ACC>
+// FZL > 0: a build for columns of exactly FZL points, C = NT * 6 / FZL per
+// workgroup, with Nay <= 2 FZL / 3.  The transform loads U straight into its
+// first butterflies and ends with the radix-3 pass in registers: lane b's
+// butterfly j (of a column s) then holds ky = j (output j) and ky = j - FZL/3
+// (output j + 2FZL/3), the only kept outputs, so every lane accumulates two
+// modes per butterfly and no LDS read-back is needed.  The passes alternate
+// between NBUF LDS buffers.
+template <typename T, int NT, int VPT, int ACC, int FZL, int NBUF>
 __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __restrict__ insts,
                                                         const StagePtrs<T>* __restrict__ ptrs, StageParams sp,
                                                         BlockMap map) {
@@ -438,11 +617,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     const int half = nay / 2;
     const int hxa = d.hxa;
     const int gx0 = tile * C;
-    const int ncol = C * L;
     const int nmodes = C * nay;
     const size_t plane = static_cast<size_t>(L) * hxa;
     const int tid = threadIdx.x;
-    const double two_pi = 2.0 * 3.14159265358979323846;
 
     // ---- flux slots this group records (FluxBuffer, src/buffer_flux.h) -------
     // Every mode is read and written by the one lane that owns it in all the
@@ -485,71 +662,141 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     // ---- forward y transforms of the products feeding this group ------------
     const bool psi_group = g.kind == GROUP_PSI;
     const bool recording = psi_group ? fx_slot != nullptr : pi_slot != nullptr;
-    for (int ci = 0; ci < g.ncontrib; ++ci) {
-        const int u = g.contrib[ci];
-        const ContribCoef& cf = g.coef[ci];
-        const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
-        const int lc = log2p2(C);
-        for (int idx = tid; idx < ncol; idx += NT) {
-            const int y = idx >> lc;
-            const int c = idx - (y << lc);
-            const int gx = gx0 + c;
-            lds[lp(c * L + y)] = gx < hxa ? src[static_cast<size_t>(y) * hxa + gx] : czero<T>();
+    // one transformed mode v of column gx, signed ky_i, compact slot cidx
+    // into accumulator slot i (and the step flux when recording)
+    auto accumulate = [&](const ContribCoef& cf, int i, cx<T> v, int gx, int ky_i, int cidx) {
+        const T kx = T(d.dkx) * T(gx);
+        const T ky = T(d.dky) * T(ky_i);
+        const T kxx = kx * kx, kyy = ky * ky, kxy = kx * ky;
+        const cx<T> mv = mul_mi(v);   // -i v
+        {
+            const T a = T(cf.a[0][0]) + T(cf.a[0][1]) * kxx + T(cf.a[0][2]) * kyy + T(cf.a[0][3]) * kxy;
+            const T b = T(cf.b[0][0]) * kx + T(cf.b[0][1]) * ky;
+            acc0[i] = acc0[i] + a * v + b * mv;
+        }
+        if (!psi_group) {
+            const T a = T(cf.a[1][0]) + T(cf.a[1][1]) * kxx + T(cf.a[1][2]) * kyy + T(cf.a[1][3]) * kxy;
+            const T b = T(cf.b[1][0]) * kx + T(cf.b[1][1]) * ky;
+            acc1[i] = acc1[i] + a * v + b * mv;
+        }
+        if (recording) {
+            const size_t at = static_cast<size_t>(gx) * nay + cidx;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                cx<T>* dst = psi_group ? (t == 0 ? fx_slot : (t == 1 ? fy_slot : nullptr)) : pi_slot + t * fe;
+                if (dst == nullptr) continue;
+                const cx<T> fv = T(cf.f[t][0]) * v + (T(cf.f[t][1]) * kx + T(cf.f[t][2]) * ky) * mv;
+                dst[at] = dst[at] + fw * fv;
+            }
+        }
+    };
+
+    cx<T>* acc_x = lds;
+    cx<T>* acc_y = lds + nmodes;
+    if constexpr (FZL > 0) {
+        constexpr int CF = NT * 6 / FZL;
+        constexpr int M3 = FZL / 3;
+        constexpr int R0 = fz_radix(FZL, false, 0);
+        constexpr int NP = fz_npass(FZL, false);
+        static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 4, "radix-3 last pass, four modes per lane");
+        static_assert(FzMap<NT, FZL, CF, 3>::NB == 2 * NT, "two radix-3 butterflies per lane");
+        using M0 = FzMap<NT, FZL, CF, R0>;
+        FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
+        for (int ci = 0; ci < g.ncontrib; ++ci) {
+            const ContribCoef& cf = g.coef[ci];
+            const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(g.contrib[ci]) * plane;
+            cx<T> v[M0::B][R0];
+            int lane0 = tid;
+            asm volatile("" : "+v"(lane0));
+#pragma unroll
+            for (int b = 0; b < M0::B; ++b) {
+                const int id = lane0 + NT * b;
+                if (!M0::valid(id)) continue;
+                const int s = id / M0::M;
+                const int j = id - s * M0::M;
+                const int gx = gx0 + s;
+                const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
+#pragma unroll
+                for (int q = 0; q < R0; ++q)   // kx-blocked U: consecutive lanes, consecutive rows
+                    v[b][q] = gx < hxa ? src[col + (static_cast<size_t>(j + q * M0::M) << d.ulg)] : czero<T>();
+            }
+            fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
+            cx<T> w[2][3];
+            fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
+            // an opaque copy of the lane index: the mode coordinates below are
+            // recomputed per contribution instead of held live across the transform
+            int lane = tid;
+            asm volatile("" : "+v"(lane));
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int id = lane + NT * b;
+                const int s = id / M3;
+                const int j = id - s * M3;
+                const int gx = gx0 + s;
+                if (gx >= hxa) continue;
+                if (j < half) accumulate(cf, 2 * b, w[b][0], gx, j, j);
+                if (M3 - j < half) accumulate(cf, 2 * b + 1, w[b][2], gx, j - M3, j - M3 + nay);
+            }
+        }
+        // park the accumulated terms at their modes once every lane is done
+        // with the transform buffers
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int id = tid + NT * b;
+            const int s = id / M3;
+            const int j = id - s * M3;
+            if (gx0 + s >= hxa) continue;
+            if (j < half) {
+                acc_x[s * nay + j] = acc0[2 * b];
+                acc_y[s * nay + j] = acc1[2 * b];
+            }
+            if (M3 - j < half) {
+                acc_x[s * nay + j - M3 + nay] = acc0[2 * b + 1];
+                acc_y[s * nay + j - M3 + nay] = acc1[2 * b + 1];
+            }
         }
         __syncthreads();
-        col_fft<T, NT, VPT>(lds, L, C, d, twy, -1);
+    } else {
+        for (int ci = 0; ci < g.ncontrib; ++ci) {
+            const int u = g.contrib[ci];
+            const ContribCoef& cf = g.coef[ci];
+            const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
+            for (int c = 0; c < C; ++c) {   // kx-blocked U: a column's rows 2^ulg apart
+                const int gx = gx0 + c;
+                const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
+                for (int y = tid; y < L; y += NT)
+                    lds[lp(c * L + y)] = gx < hxa ? src[col + (static_cast<size_t>(y) << d.ulg)] : czero<T>();
+            }
+            __syncthreads();
+            col_fft<T, NT, VPT>(lds, L, C, d, twy, -1);
+#pragma unroll
+            for (int i = 0; i < ACC; ++i) {
+                const int m = tid + NT * i;
+                if (m >= nmodes) continue;
+                const int c = divn(m, nay, d.lnay);
+                const int cidx = m - c * nay;
+                const int gx = gx0 + c;
+                if (gx >= hxa || cidx == half) continue;
+                const int ky_i = cidx < half ? cidx : cidx - nay;
+                const int gy = ky_i >= 0 ? ky_i : ky_i + L;
+                accumulate(cf, i, lds[lp(c * L + gy)], gx, ky_i, cidx);
+            }
+            __syncthreads();
+        }
+
+        // ---- park the accumulated product terms in LDS (free after the last
+        // barrier) so the stage update below can walk the modes in a plain loop
 #pragma unroll
         for (int i = 0; i < ACC; ++i) {
             const int m = tid + NT * i;
-            if (m >= nmodes) continue;
-            const int c = divn(m, nay, d.lnay);
-            const int cidx = m - c * nay;
-            const int gx = gx0 + c;
-            if (gx >= hxa || cidx == half) continue;
-            const int ky_i = cidx < half ? cidx : cidx - nay;
-            const int gy = ky_i >= 0 ? ky_i : ky_i + L;
-            const cx<T> v = lds[lp(c * L + gy)];
-            const T kx = T(d.dkx) * T(gx);
-            const T ky = T(d.dky) * T(ky_i);
-            const T kxx = kx * kx, kyy = ky * ky, kxy = kx * ky;
-            const cx<T> mv = mul_mi(v);   // -i v
-            {
-                const T a = T(cf.a[0][0]) + T(cf.a[0][1]) * kxx + T(cf.a[0][2]) * kyy + T(cf.a[0][3]) * kxy;
-                const T b = T(cf.b[0][0]) * kx + T(cf.b[0][1]) * ky;
-                acc0[i] = acc0[i] + a * v + b * mv;
-            }
-            if (!psi_group) {
-                const T a = T(cf.a[1][0]) + T(cf.a[1][1]) * kxx + T(cf.a[1][2]) * kyy + T(cf.a[1][3]) * kxy;
-                const T b = T(cf.b[1][0]) * kx + T(cf.b[1][1]) * ky;
-                acc1[i] = acc1[i] + a * v + b * mv;
-            }
-            if (recording) {
-                const size_t at = static_cast<size_t>(gx) * nay + cidx;
-#pragma unroll
-                for (int t = 0; t < 3; ++t) {
-                    cx<T>* dst = psi_group ? (t == 0 ? fx_slot : (t == 1 ? fy_slot : nullptr)) : pi_slot + t * fe;
-                    if (dst == nullptr) continue;
-                    const cx<T> fv = T(cf.f[t][0]) * v + (T(cf.f[t][1]) * kx + T(cf.f[t][2]) * ky) * mv;
-                    dst[at] = dst[at] + fw * fv;
-                }
+            if (m < nmodes) {
+                acc_x[m] = acc0[i];
+                acc_y[m] = acc1[i];
             }
         }
         __syncthreads();
     }
-
-    // ---- park the accumulated product terms in LDS (free after the last
-    // barrier) so the stage update below can walk the modes in a plain loop
-    cx<T>* acc_x = lds;
-    cx<T>* acc_y = lds + nmodes;
-#pragma unroll
-    for (int i = 0; i < ACC; ++i) {
-        const int m = tid + NT * i;
-        if (m < nmodes) {
-            acc_x[m] = acc0[i];
-            acc_y[m] = acc1[i];
-        }
-    }
-    __syncthreads();
 
     // ---- spectral terms, noise, stage update --------------------------------
     const int a = g.alpha;

@@ -43,17 +43,21 @@ std::string g_create_error;
 constexpr int kClasses = 3;
 constexpr size_t kLdsCap = 96 * 1024;
 constexpr size_t kLdsCap3 = 128 * 1024;
+constexpr size_t kLdsMax = 160 * 1024;   // LDS of one CU
 constexpr int kMaxSlots = 4096;
 
 struct ColPlan {
     int C = 1, NT = 256, V = 6, ntiles = 1;
     int acc = 6;       // K3 accumulator slots per lane: 4 when C*Nay <= 4*NT, else V
+    int fz = 0;        // column length of the fused build in use, 0 for the staged one
+    int nbuf = 1;      // LDS buffers of the fused build
     size_t lds = 0;    // K1: padded C*My values
     size_t lds3 = 0;   // K3: max(padded C*My, 2*C*Nay) values (FFT column, then both accumulators)
 };
 struct RowPlan {
     int R = 1, NT = 256, V = 6, nblocks = 1, maxn = 0;
     int fl = 0;        // row length of a fixed-length row-kernel build, 0 for the any-length one
+    int nbuf = 0;      // fixed-length build: 0 staged, 1 or 2 fused with that many LDS buffers
     size_t lds = 0;
 };
 
@@ -150,19 +154,44 @@ using K2Fn = void (*)(const InstDev<T>*, BlockMap);
 template <typename T>
 using K3Fn = void (*)(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
 
+// fused column builds exist for (NT, L)
+bool col_fused_built(int NT, int L) {
+#define SHHD_CFZ(nt, len) if (NT == nt && L == len) return true;
+    SHHD_FOR_COL_FZ(SHHD_CFZ)
+#undef SHHD_CFZ
+    return false;
+}
+
 template <typename T>
-K1Fn<T> pick_k1(int NT, int V) {
-#define SHHD_K1(nt, v) if (NT == nt && V == v) return &k_inverse_columns<T, nt, v>;
+K1Fn<T> pick_k1(int NT, int V, int fz, int nbuf) {
+    if (fz > 0) {
+#define SHHD_K1FZ(nt, len)                                                                 \
+    if (NT == nt && fz == len) return nbuf == 2 ? &k_inverse_columns<T, nt, 6, len, 2> \
+                                                : &k_inverse_columns<T, nt, 6, len, 1>;
+        SHHD_FOR_COL_FZ(SHHD_K1FZ)
+#undef SHHD_K1FZ
+        return nullptr;
+    }
+#define SHHD_K1(nt, v) if (NT == nt && V == v) return &k_inverse_columns<T, nt, v, 0, 1>;
     SHHD_FOR_NT_V(SHHD_K1)
 #undef SHHD_K1
     return nullptr;
 }
 
 template <typename T>
-K3Fn<T> pick_k3(int NT, int V, int acc) {
+K3Fn<T> pick_k3(int NT, int V, int acc, int fz, int nbuf) {
+    if (fz > 0) {
+        if (acc != 4) return nullptr;
+#define SHHD_K3FZ(nt, len)                                                                    \
+    if (NT == nt && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, 6, 4, len, 2> \
+                                                : &k_forward_columns<T, nt, 6, 4, len, 1>;
+        SHHD_FOR_COL_FZ(SHHD_K3FZ)
+#undef SHHD_K3FZ
+        return nullptr;
+    }
 #define SHHD_K3(nt, v) \
-    if (NT == nt && V == v && acc == 4) return &k_forward_columns<T, nt, v, 4>; \
-    if (NT == nt && V == v && acc == v) return &k_forward_columns<T, nt, v, v>;
+    if (NT == nt && V == v && acc == 4) return &k_forward_columns<T, nt, v, 4, 0, 1>; \
+    if (NT == nt && V == v && acc == v) return &k_forward_columns<T, nt, v, v, 0, 1>;
     SHHD_FOR_NT_V(SHHD_K3)
 #undef SHHD_K3
     return nullptr;
@@ -177,17 +206,19 @@ int row_fixed_length(int mx, int NT, int V) {
 }
 
 template <typename T>
-K2Fn<T> pick_k2(int NT, int V, int maxn, int fl) {
+K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf) {
     const int n = maxn <= 0 ? 0 : (maxn == 1 ? 1 : (maxn == 2 ? 2 : (maxn <= 4 ? 4 : 8)));
     if (fl > 0) {
-#define SHHD_K2FN(nt, v, mn, len) if (NT == nt && V == v && n == mn && fl == len) return &k_rows<T, nt, v, mn, len>;
-#define SHHD_K2F(nt, v, len) SHHD_K2FN(nt, v, 0, len) SHHD_K2FN(nt, v, 1, len) SHHD_K2FN(nt, v, 2, len) \
-    SHHD_K2FN(nt, v, 4, len) SHHD_K2FN(nt, v, 8, len)
+#define SHHD_K2FN(nt, v, len, mn)                                                  \
+    if (NT == nt && V == v && n == mn && fl == len)                                \
+        return nbuf == 2 ? &k_rows<T, nt, v, mn, len, 2>                           \
+                         : (nbuf == 1 ? &k_rows<T, nt, v, mn, len, 1> : &k_rows<T, nt, v, mn, len, 0>);
+#define SHHD_K2F(nt, v, len) SHHD_FOR_MAXN(SHHD_K2FN, nt, v, len)
         SHHD_FOR_ROW_FL(SHHD_K2F)
 #undef SHHD_K2F
 #undef SHHD_K2FN
     }
-#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0>;
+#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0>;
 #define SHHD_K2(nt, v) SHHD_K2N(nt, v, 0) SHHD_K2N(nt, v, 1) SHHD_K2N(nt, v, 2) SHHD_K2N(nt, v, 4) SHHD_K2N(nt, v, 8)
     SHHD_FOR_NT_V(SHHD_K2)
 #undef SHHD_K2
@@ -721,9 +752,14 @@ shhd_status validate(const shhd_config& c) {
 struct Tuning {
     int c1 = 0, c3 = 0, rnt = 0;   // 0: the untuned default
     // transforms: 0 fixed-length build where one exists (default), 1 runtime
-    // plan with radix-8 passes first, 2 runtime plan with radix-4 passes
+    // plan with radix-8 passes first, 2 runtime plan with radix-4 passes,
+    // 3 and 4 fused fixed-length build with one and two LDS buffers (else 0)
     int col_transform = 0, row_transform = 0;
+    // U planes in blocks of 2^ulg kx columns (0: column-major); a block
+    // that does not divide Nax/2 falls back to 0
+    int ulg = 0;
 };
+constexpr int kTransforms = 5;
 std::mutex g_tune_mutex;
 std::map<std::string, Tuning> g_tuned;
 
@@ -764,8 +800,27 @@ void apply_plans(Instance& h, const Tuning& t) {
     h.cols = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 32, t.c1);
     h.cols3 = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 16, t.c3);
     h.rows = plan_rows(d.mx, d.my, h.elem, std::max(nt_min, t.rnt));
-    if (t.row_transform != 0) h.rows.fl = 0;
-    h.desc.fixed_y = t.col_transform == 0 ? 1 : 0;
+    if (t.row_transform == 1 || t.row_transform == 2) h.rows.fl = 0;
+    if (t.row_transform >= 3 && h.rows.fl > 0) {
+        const size_t one = static_cast<size_t>(lp_size(h.rows.R * d.mx)) * h.elem;
+        h.rows.nbuf = (t.row_transform == 4 && 2 * one <= kLdsMax) ? 2 : 1;
+        h.rows.lds = h.rows.nbuf * one;
+    }
+    // fused column builds: C * My = NT * 6 and (K3) every kept ky within the
+    // first and last thirds of the column
+    const int fz_nb = t.col_transform == 3 ? 1 : (t.col_transform == 4 ? 2 : 0);
+    if (fz_nb > 0 && d.my % 3 == 0 && 3 * d.nay <= 2 * d.my) {
+        for (ColPlan* p : {&h.cols, &h.cols3}) {
+            if (p->NT * 6 != p->C * d.my || !col_fused_built(p->NT, d.my) || (p == &h.cols3 && p->acc != 4)) continue;
+            const size_t one = static_cast<size_t>(lp_size(p->C * d.my)) * h.elem;
+            p->fz = d.my;
+            p->nbuf = (fz_nb == 2 && 2 * one <= kLdsMax) ? 2 : 1;
+            p->lds = p->nbuf * one;
+            p->lds3 = std::max(p->nbuf * one, static_cast<size_t>(2 * p->C * d.nay) * h.elem);
+        }
+    }
+    h.desc.ulg = (t.ulg > 0 && t.ulg <= 12 && d.hxa % (1 << t.ulg) == 0) ? t.ulg : 0;
+    h.desc.fixed_y = (t.col_transform == 0 || t.col_transform >= 3) ? 1 : 0;
     h.desc.plan_y = make_fft_plan(d.my, t.col_transform == 2 ? 4 : 8);
     h.desc.plan_x = make_fft_plan(d.mx, t.row_transform == 2 ? 4 : 8);
     int npsi_roles = 0;
@@ -956,9 +1011,11 @@ shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t
 struct ClassKey {
     bool fp32;
     int stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl;
+    int cfz, cnb, c3fz, c3nb, rnb;   // fused builds (length, LDS buffers)
     bool operator<(const ClassKey& o) const {
-        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl) <
-               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.c3NT, o.c3V, o.acc, o.rNT, o.rV, o.maxn, o.rfl);
+        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl, cfz, cnb, c3fz, c3nb, rnb) <
+               std::tie(o.fp32, o.stages, o.cNT, o.cV, o.c3NT, o.c3V, o.acc, o.rNT, o.rV, o.maxn, o.rfl, o.cfz, o.cnb,
+                        o.c3fz, o.c3nb, o.rnb);
     }
 };
 
@@ -1446,7 +1503,7 @@ struct Engine {
                 (e = hipMemcpy(dmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) {
                 fail(e, "hipMemcpy");
             } else {
-                K1Fn<T> k1 = pick_k1<T>(h.cols.NT, h.cols.V);
+                K1Fn<T> k1 = pick_k1<T>(h.cols.NT, h.cols.V, h.cols.fz, h.cols.nbuf);
                 KPFn<T> kp = pick_kp<T>(h.rows.NT, h.rows.V);
                 (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k1), hipFuncAttributeMaxDynamicSharedMemorySize,
                                           static_cast<int>(h.cols.lds));
@@ -1638,11 +1695,11 @@ struct Engine {
             SHHD_HIP(&err, hipMemcpy(lc.d_static, statics.data(), statics.size() * sizeof(StaticEntry<T>), hipMemcpyHostToDevice));
         }
         const ClassKey& K = lc.key;
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV, K.cfz, K.cnb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
         return SHHD_OK;
     }
@@ -1656,8 +1713,9 @@ struct Engine {
         for (int s = 0; s < kMaxSlots; ++s) {
             const Instance* h = slots[static_cast<size_t>(s)].get();
             if (!h) continue;
-            ClassKey k{h->fp32, h->stages, h->cols.NT, h->cols.V, h->cols3.NT, h->cols3.V, h->cols3.acc, h->rows.NT,
-                       h->rows.V, h->rows.maxn, h->rows.fl};
+            ClassKey k{h->fp32,      h->stages,     h->cols.NT,     h->cols.V,       h->cols3.NT, h->cols3.V,
+                       h->cols3.acc, h->rows.NT,    h->rows.V,      h->rows.maxn,    h->rows.fl,  h->cols.fz,
+                       h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf};
             groups[k].push_back(s);
         }
         for (auto& g : groups) {
@@ -1711,20 +1769,20 @@ struct Engine {
         const StagePtrs<T>* ptrs = static_cast<const StagePtrs<T>*>(lc.d_ptrs) + static_cast<size_t>(k * K.stages + s) * n;
         shhd_status st = SHHD_OK;
         if (lc.nblk[0] > 0) {
-            K1Fn<T> k1 = pick_k1<T>(K.cNT, K.cV);
+            K1Fn<T> k1 = pick_k1<T>(K.cNT, K.cV, K.cfz, K.cnb);
             st = timed(0, capturing, [&] {
                 hipLaunchKernelGGL(k1, dim3(lc.nblk[0]), dim3(K.cNT), lc.lds[0], stream, inst, ptrs, lc.map[0]);
             });
             if (st != SHHD_OK) return st;
         }
         if (lc.nblk[1] > 0) {
-            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl);
+            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb);
             st = timed(1, capturing, [&] {
                 hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], stream, inst, lc.map[1]);
             });
             if (st != SHHD_OK) return st;
         }
-        K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc);
+        K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb);
         return timed(2, capturing, [&] {
             hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.c3NT), lc.lds[2], stream, inst, ptrs, sp, lc.map[2]);
         });
@@ -1955,7 +2013,7 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
     cur.c3 = best.c3;
     // column transforms (shared by both column passes: their summed time)
     bk = 1e30;
-    for (int ct : {0, 1, 2}) {
+    for (int ct = 0; ct < kTransforms; ++ct) {
         Tuning t = cur;
         t.col_transform = ct;
         if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
@@ -1972,11 +2030,22 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
     }
     cur.rnt = best.rnt;
     bk = 1e30;
-    for (int rt : {0, 1, 2}) {
+    for (int rt = 0; rt < kTransforms; ++rt) {
         Tuning t = cur;
         t.row_transform = rt;
         if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
         if (ms[1] < bk) { bk = ms[1]; best.row_transform = rt; best_ms[1] = ms[1]; }
+    }
+    cur.row_transform = best.row_transform;
+    // kx blocks of the row-to-column planes: the row pass writes them, the
+    // forward column pass reads them (their summed time)
+    bk = 1e30;
+    for (int ulg = 0; ulg < 4; ++ulg) {
+        if (d.hxa % (1 << ulg) != 0) break;
+        Tuning t = cur;
+        t.ulg = ulg;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[1] + ms[2] < bk) { bk = ms[1] + ms[2]; best.ulg = ulg; best_ms[1] = ms[1]; best_ms[2] = ms[2]; }
     }
     store_tuning(h.tune_key, best);
     return SHHD_OK;
@@ -2059,6 +2128,7 @@ shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out
     out->row_lanes = best.rnt;
     out->column_transform = best.col_transform;
     out->row_transform = best.row_transform;
+    out->plane_block = best.ulg;
     for (int k = 0; k < kClasses; ++k) out->ms[k] = ms[k];
     return SHHD_OK;
 }
@@ -2071,10 +2141,11 @@ shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* t) {
     if (st != SHHD_OK) return st;
     if (t->inverse_column_tile < 0 || t->forward_column_tile < 0 ||
         (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024) ||
-        t->column_transform < 0 || t->column_transform > 2 || t->row_transform < 0 || t->row_transform > 2)
+        t->column_transform < 0 || t->column_transform >= kTransforms || t->row_transform < 0 ||
+        t->row_transform >= kTransforms || t->plane_block < 0 || t->plane_block > 12)
         return set_err(nullptr, SHHD_ERR_INVALID, "invalid launch shapes");
     store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes, t->column_transform,
-                                     t->row_transform});
+                                     t->row_transform, t->plane_block});
     return SHHD_OK;
 }
 
@@ -2217,6 +2288,12 @@ shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info) {
     info->stages_per_step = I->stages;
     info->state_doubles = 2 * ref_spec(I->cfg) * static_cast<size_t>(I->desc.nfields);
     info->device_bytes = I->bytes;
+    info->inverse_column_tile = I->cols.C;
+    info->forward_column_tile = I->cols3.C;
+    info->row_lanes = I->rows.NT;
+    info->inverse_column_fused = I->cols.fz > 0 ? I->cols.nbuf : 0;
+    info->forward_column_fused = I->cols3.fz > 0 ? I->cols3.nbuf : 0;
+    info->row_fused = I->rows.nbuf;
     return SHHD_OK;
 }
 
