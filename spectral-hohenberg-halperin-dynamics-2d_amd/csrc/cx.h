@@ -32,6 +32,32 @@ template <typename T> SHHD_HD cx<T> mul_i(cx<T> a) { return mk(-a.im, a.re); }
 template <typename T> SHHD_HD cx<T> mul_mi(cx<T> a) { return mk(a.im, -a.re); }
 template <typename T> SHHD_HD cx<T> czero() { return mk(T(0), T(0)); }
 
+#if defined(__HIPCC__)
+// Loads and stores of device (global) memory through pointers the compiler
+// cannot place in an address space (they come out of descriptor tables):
+// global_load / global_store with a scalar base and a 32-bit lane offset
+// instead of flat accesses with 64-bit lane addresses, which also count
+// against the LDS wait counter.
+template <typename T> struct vec2_of;
+template <> struct vec2_of<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct vec2_of<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+template <typename T>
+__device__ __forceinline__ cx<T> gload(const cx<T>* p) {
+    using V = typename vec2_of<T>::type;
+    const V v = *(const __attribute__((address_space(1))) V*)(p);
+    return mk(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cx<T>* p, cx<T> z) {
+    using V = typename vec2_of<T>::type;
+    V v;
+    v.x = z.re;
+    v.y = z.im;
+    *(__attribute__((address_space(1))) V*)(p) = v;
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
 // as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the

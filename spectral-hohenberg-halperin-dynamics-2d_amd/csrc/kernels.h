@@ -106,7 +106,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
                 v[b][q] = czero<T>();
                 if (gx < d.hxa && ky < half && -ky < half) {
                     const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v[b][q] = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+                    v[b][q] = scale * gload(src + static_cast<size_t>(gx) * d.nay + cidx);
                 }
             }
         }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
                 const int ky = signed_gy(y, L);
                 if (gx < d.hxa && ky < half && -ky < half) {
                     const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v = scale * src[static_cast<size_t>(gx) * d.nay + cidx];
+                    v = scale * gload(src + static_cast<size_t>(gx) * d.nay + cidx);
                 }
                 lds[lp(c * L + y)] = v;
             }
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         const int y = idx >> lc;
         const int c = idx - (y << lc);
         const int gx = gx0 + c;
-        if (gx < d.hxa) dst[static_cast<size_t>(y) * d.hxa + gx] = res[lp(c * L + y)];
+        if (gx < d.hxa) gstore(dst + static_cast<size_t>(y) * d.hxa + gx, res[lp(c * L + y)]);
     }
 }
 
@@ -302,8 +302,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     const int kk = mirror ? FL - k : k;
                     cx<T> z = czero<T>();
                     if (y < my && (k < hxa || mirror)) {
-                        cx<T> a = ta[row + kk];
-                        cx<T> bb = has_b ? tb[row + kk] : czero<T>();
+                        cx<T> a = gload(ta + row + kk);
+                        cx<T> bb = has_b ? gload(tb + row + kk) : czero<T>();
                         if (k == 0) { a.im = T(0); bb.im = T(0); }
                         if (mirror) { a.im = -a.im; bb.im = -bb.im; }   // conj(A) + i conj(B)
                         z = mk(a.re - bb.im, a.im + bb.re);             // A + iB
@@ -370,8 +370,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     const cx<T> zk = res[lp(r * FL + k)];
                     const cx<T> zm = res[lp(r * FL + (k == 0 ? 0 : FL - k))];
                     const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);
-                    ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
-                    if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
+                    gstore(ua + at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
+                    if (has_b) gstore(ub + at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
                 }
             }
         }
@@ -391,14 +391,14 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             if (y < my) {
                 const size_t row = static_cast<size_t>(y) * hxa;
                 if (k < hxa) {
-                    cx<T> a = ta[row + k];
-                    cx<T> b = has_b ? tb[row + k] : czero<T>();
+                    cx<T> a = gload(ta + row + k);
+                    cx<T> b = has_b ? gload(tb + row + k) : czero<T>();
                     if (k == 0) { a.im = T(0); b.im = T(0); }
                     z = mk(a.re - b.im, a.im + b.re);               // A + iB
                 } else if (k > L - hxa) {
                     const int kk = L - k;
-                    const cx<T> a = ta[row + kk];
-                    const cx<T> b = has_b ? tb[row + kk] : czero<T>();
+                    const cx<T> a = gload(ta + row + kk);
+                    const cx<T> b = has_b ? gload(tb + row + kk) : czero<T>();
                     z = mk(a.re + b.im, b.re - a.im);               // conj(A) + i conj(B)
                 }
             }
@@ -462,8 +462,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 const cx<T> zk = lds[lp(r * L + k)];
                 const cx<T> zm = lds[lp(r * L + (k == 0 ? 0 : L - k))];
                 const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);   // kx-blocked U
-                ua[at] = mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im));
-                if (has_b) ub[at] = mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re));
+                gstore(ua + at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
+                if (has_b) gstore(ub + at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
             }
         }
         __syncthreads();
@@ -504,14 +504,14 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
             if (y < my) {
                 const size_t row = static_cast<size_t>(y) * hxa;
                 if (k < hxa) {
-                    cx<T> a = ta[row + k];
-                    cx<T> b = has_b ? tb[row + k] : czero<T>();
+                    cx<T> a = gload(ta + row + k);
+                    cx<T> b = has_b ? gload(tb + row + k) : czero<T>();
                     if (k == 0) { a.im = T(0); b.im = T(0); }
                     z = mk(a.re - b.im, a.im + b.re);
                 } else if (k > L - hxa) {
                     const int kk = L - k;
-                    const cx<T> a = ta[row + kk];
-                    const cx<T> b = has_b ? tb[row + kk] : czero<T>();
+                    const cx<T> a = gload(ta + row + kk);
+                    const cx<T> b = has_b ? gload(tb + row + kk) : czero<T>();
                     z = mk(a.re + b.im, b.re - a.im);
                 }
             }
@@ -534,34 +534,32 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
 }
 
 // ---------------------------------------------------------------------------
-// noise: one complex N(0,1) pair per Philox call
+// noise: two complex N(0,1) deviates per Philox call.  The four 32-bit words
+// give two Box-Muller pairs, each from a radius uniform u1 = (r + 1/2) 2^-32
+// (so the tails reach sqrt(-2 ln 2^-33) = 6.8 sigma) and an angle uniform;
+// the transcendentals run in single precision in both arithmetic types
+// (the deviates' law is the normal law to ~1e-7, far below any sampling
+// error a run can resolve).
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ cx<T> gauss_pair(const Desc& d, uint32_t mode_id, uint32_t stream, uint32_t draw,
-                                            const StageParams& sp) {
+__device__ __forceinline__ void gauss_quad(const Desc& d, uint32_t mode_id, uint32_t stream, const StageParams& sp,
+                                           cx<T>& z0, cx<T>& z1) {
     u32x4 ctr;
     ctr.v[0] = mode_id;
-    ctr.v[1] = (stream << 4) | draw;
+    ctr.v[1] = stream << 4;
     ctr.v[2] = sp.step_lo;
     ctr.v[3] = sp.step_hi;
     const u32x4 r = philox4x32_10(ctr, d.seed_lo, d.seed_hi);
-    if constexpr (sizeof(T) == 8) {
-        const uint64_t a = (static_cast<uint64_t>(r.v[0]) << 21) | (r.v[1] >> 11);
-        const uint64_t b = (static_cast<uint64_t>(r.v[2]) << 21) | (r.v[3] >> 11);
-        const double u1 = (static_cast<double>(a) + 0.5) * 0x1.0p-53;
-        const double u2 = (static_cast<double>(b) + 0.5) * 0x1.0p-53;
-        const double rad = sqrt(-2.0 * log(u1));
-        double s, c;
-        sincospi(2.0 * u2, &s, &c);
-        return mk(rad * c, rad * s);
-    } else {
-        const float u1 = (static_cast<float>(r.v[0] >> 8) + 0.5f) * 0x1.0p-24f;
-        const float u2 = (static_cast<float>(r.v[1] >> 8) + 0.5f) * 0x1.0p-24f;
+    auto box_muller = [](uint32_t a, uint32_t b) {
+        const float u1 = (static_cast<float>(a) + 0.5f) * 0x1.0p-32f;
+        const float u2 = static_cast<float>(b) * 0x1.0p-32f;
         const float rad = sqrtf(-2.0f * logf(u1));
-        float s, c;
-        sincospif(2.0f * u2, &s, &c);
-        return mk(rad * c, rad * s);
-    }
+        float sn, cs;
+        sincospif(2.0f * u2, &sn, &cs);
+        return mk(static_cast<T>(rad * cs), static_cast<T>(rad * sn));
+    };
+    z0 = box_muller(r.v[0], r.v[1]);
+    z1 = box_muller(r.v[2], r.v[3]);
 }
 
 // FP32 form: the k = 0 mode is advanced in FP64 from its FP64 copies; its
@@ -718,7 +716,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
 #pragma unroll
                 for (int q = 0; q < R0; ++q)   // kx-blocked U: consecutive lanes, consecutive rows
-                    v[b][q] = gx < hxa ? src[col + (static_cast<size_t>(j + q * M0::M) << d.ulg)] : czero<T>();
+                    v[b][q] = gx < hxa ? gload(src + col + (static_cast<size_t>(j + q * M0::M) << d.ulg)) : czero<T>();
             }
             fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
             cx<T> w[2][3];
@@ -766,7 +764,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 const int gx = gx0 + c;
                 const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
                 for (int y = tid; y < L; y += NT)
-                    lds[lp(c * L + y)] = gx < hxa ? src[col + (static_cast<size_t>(y) << d.ulg)] : czero<T>();
+                    lds[lp(c * L + y)] = gx < hxa ? gload(src + col + (static_cast<size_t>(y) << d.ulg)) : czero<T>();
             }
             __syncthreads();
             col_fft<T, NT, VPT>(lds, L, C, d, twy, -1);
@@ -814,11 +812,11 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         const size_t at = static_cast<size_t>(gx) * nay + cidx;
         if (cidx == half) {   // the always-zero ky = -Nay/2 slot
             if (psi_group) {
-                P.next[1 + a][at] = czero<T>();
+                gstore(P.next[1 + a] + at, czero<T>());
             } else {
-                if (compressible) P.next[0][at] = czero<T>();
-                P.next[fx_field][at] = czero<T>();
-                P.next[fy_field][at] = czero<T>();
+                if (compressible) gstore(P.next[0] + at, czero<T>());
+                gstore(P.next[fx_field] + at, czero<T>());
+                gstore(P.next[fy_field] + at, czero<T>());
             }
             continue;
         }
@@ -835,7 +833,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
 
         if (psi_group) {
             const int f = 1 + a;
-            const cx<T> cur = P.cur[f][at];
+            const cx<T> cur = gload(P.cur[f] + at);
             cx<T> det = acc_x[m];
             if (d.psi_det_on[a]) {
                 if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
@@ -851,7 +849,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
                 for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
                     if (d.igrad_component[q] != a) continue;
-                    const cx<T> j = P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field][at];
+                    const cx<T> j = gload(P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field] + at);
                     const cx<T> t = T(-d.igrad_amplitude[q]) * j;
                     det = det + mk(t.re / T(d.rho0), t.im / T(d.rho0));
                 }
@@ -868,16 +866,17 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 cx<T> sa, sb = czero<T>();
                 cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
                 if (sp.noise_mode == NOISE_LOAD) {
-                    sa = nz[at];
-                    sb = nz[fe + at];
+                    sa = gload(nz + at);
+                    sb = gload(nz + fe + at);
                 } else {
                     const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
-                    sa = sigma * gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 0u, sp);
-                    if (sp.use_b) sb = sigma * gauss_pair<T>(d, mode_id, static_cast<uint32_t>(a), 1u, sp);
+                    gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
+                    sa = sigma * sa;
+                    sb = sp.use_b ? sigma * sb : czero<T>();
                     if (conj_draw) { sa = conj(sa); sb = conj(sb); }
                     if (sp.noise_mode == NOISE_STORE) {
-                        nz[at] = sa;
-                        nz[fe + at] = sb;
+                        gstore(nz + at, sa);
+                        gstore(nz + fe + at, sb);
                     }
                 }
                 sto = sp.use_b ? sa + beta * sb : sa;
@@ -888,14 +887,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
             }
             if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
-            else P.next[f][at] = wb * P.base[f][at] + wc * (cur + dt * det + sdt * sto);
+            else gstore(P.next[f] + at, wb * gload(P.base[f] + at) + wc * (cur + dt * det + sdt * sto));
         } else {
-            const cx<T> jx = P.cur[fx_field][at];
-            const cx<T> jy = P.cur[fy_field][at];
+            const cx<T> jx = gload(P.cur[fx_field] + at);
+            const cx<T> jy = gload(P.cur[fy_field] + at);
             cx<T> dx = acc_x[m], dy = acc_y[m];
             if (d.mom_det_on) {
                 if (d.pressure_coefficient != 0.0) {   // src/fcalculator.cc:276-300
-                    const cx<T> p = T(d.pressure_coefficient) * P.cur[0][at];
+                    const cx<T> p = T(d.pressure_coefficient) * gload(P.cur[0] + at);
                     dx = dx + mul_mi(kx * p);
                     dy = dy + mul_mi(ky * p);
                     if (pi_slot) {
@@ -929,17 +928,19 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             cx<T> sx = czero<T>(), sy = czero<T>();
             cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
             if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
-                sx = nzj[at] + beta * nzj[2 * fe + at];
-                sy = nzj[fe + at] + beta * nzj[3 * fe + at];
+                sx = gload(nzj + at) + beta * gload(nzj + 2 * fe + at);
+                sy = gload(nzj + fe + at) + beta * gload(nzj + 3 * fe + at);
             } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
                 const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
                 const T kxd = conj_draw ? T(0) : kx;
                 const T kyd = conj_draw ? -ky : ky;
                 const int ndraw = sp.use_b ? 2 : 1;
+                cx<T> x3a, x3b;   // the xy deviates of draws A and B
+                gauss_quad<T>(d, mode_id, 10u, sp, x3a, x3b);
                 for (int ab = 0; ab < ndraw; ++ab) {
-                    const cx<T> x1 = gauss_pair<T>(d, mode_id, 8u, static_cast<uint32_t>(ab), sp);
-                    const cx<T> x2 = gauss_pair<T>(d, mode_id, 9u, static_cast<uint32_t>(ab), sp);
-                    const cx<T> x3 = gauss_pair<T>(d, mode_id, 10u, static_cast<uint32_t>(ab), sp);
+                    cx<T> x1, x2;
+                    gauss_quad<T>(d, mode_id, 8u + static_cast<uint32_t>(ab), sp, x1, x2);
+                    const cx<T> x3 = ab == 0 ? x3a : x3b;
                     const cx<T> pxx = -ss * (sz * x1 + se * x2);
                     const cx<T> pyy = -ss * (sz * x1 - se * x2);
                     const cx<T> pxy = (-ss * se) * x3;
@@ -953,8 +954,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                         pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
                     }
                     if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
-                        nzj[static_cast<size_t>(2 * ab) * fe + at] = vx;
-                        nzj[static_cast<size_t>(2 * ab + 1) * fe + at] = vy;
+                        gstore(nzj + static_cast<size_t>(2 * ab) * fe + at, vx);
+                        gstore(nzj + static_cast<size_t>(2 * ab + 1) * fe + at, vy);
                     }
                     if (ab == 0) { sx = vx; sy = vy; }
                     else { sx = sx + beta * vx; sy = sy + beta * vy; }
@@ -982,9 +983,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
                 } else {
-                    P.next[0][at] = wb * P.base[0][at] + wc * (P.cur[0][at] + dt * drho);
-                    P.next[fx_field][at] = wb * P.base[fx_field][at] + wc * (jx + dt * dx + sdt * sx);
-                    P.next[fy_field][at] = wb * P.base[fy_field][at] + wc * (jy + dt * dy + sdt * sy);
+                    gstore(P.next[0] + at, wb * gload(P.base[0] + at) + wc * (gload(P.cur[0] + at) + dt * drho));
+                    gstore(P.next[fx_field] + at, wb * gload(P.base[fx_field] + at) + wc * (jx + dt * dx + sdt * sx));
+                    gstore(P.next[fy_field] + at, wb * gload(P.base[fy_field] + at) + wc * (jy + dt * dy + sdt * sy));
                 }
             } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
                 cx<T> ix = dt * dx + sdt * sx;
@@ -1000,8 +1001,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
                 } else {
-                    P.next[fx_field][at] = wb * P.base[fx_field][at] + wc * (jx + ix);
-                    P.next[fy_field][at] = wb * P.base[fy_field][at] + wc * (jy + iy);
+                    gstore(P.next[fx_field] + at, wb * gload(P.base[fx_field] + at) + wc * (jx + ix));
+                    gstore(P.next[fy_field] + at, wb * gload(P.base[fy_field] + at) + wc * (jy + iy));
                 }
             }
         }

@@ -95,7 +95,7 @@ __device__ __forceinline__ void dft_small(cx<T> (&v)[R], int sign) {
 template <typename T, int R>
 __device__ __forceinline__ void apply_twiddles(cx<T> (&v)[R], const cx<T>* __restrict__ tw, int t, int sign) {
     auto ld = [&](int e) {
-        cx<T> w = tw[e];
+        cx<T> w = gload(tw + e);
         if (sign > 0) w.im = -w.im;
         return w;
     };

@@ -829,13 +829,14 @@ void apply_plans(Instance& h, const Tuning& t) {
     h.rows.maxn = npsi_roles == 0 ? 0 : (npsi_roles == 1 ? 1 : (npsi_roles == 2 ? 2 : (npsi_roles <= 4 ? 4 : 8)));
 }
 
-// SRK3 with noise keeps the step's draws A and B for stages 2 and 3:
-// 2 fields per order parameter and 4 for the momentum
+// SRK3 with noise in the FP32 form keeps the step's draws A and B for
+// stages 2 and 3: 2 fields per order parameter and 4 for the momentum (the
+// FP64 form draws them again)
 size_t noise_elems(const Instance& h) {
     const Desc& d = h.desc;
     bool on = d.j_noise_on != 0;
     for (int a = 0; a < d.N; ++a) on = on || d.psi_noise_on[a] != 0;
-    if (h.stages != 3 || !on) return 0;
+    if (h.stages != 3 || !on || !h.fp32) return 0;
     return static_cast<size_t>(2 * d.N + 4) * h.field_elems;
 }
 
@@ -1826,7 +1827,10 @@ struct Engine {
                     sp.flux_w = 1.0;
                 }
                 sp.flux_first = s == 0 ? 1 : 0;
-                sp.noise_mode = lc.key.stages == 3 ? (s == 0 ? NOISE_STORE : NOISE_LOAD) : NOISE_DRAW;
+                // SRK3 draws A and B once per step: the FP32 form keeps them for
+                // stages 2 and 3, the FP64 form draws them again (the counter-based
+                // generator gives the same values; measured faster than the cache)
+                sp.noise_mode = (lc.key.stages == 3 && lc.key.fp32) ? (s == 0 ? NOISE_STORE : NOISE_LOAD) : NOISE_DRAW;
                 sp.flux_last = s == lc.key.stages - 1 ? 1 : 0;
                 shhd_status st = lc.key.fp32 ? launch_stage<float>(lc, k, s, sp, capturing)
                                              : launch_stage<double>(lc, k, s, sp, capturing);
