@@ -130,6 +130,7 @@ typedef struct shhd_info {
      * HBM into registers, LDS exchanges over that many buffers) */
     int inverse_column_tile, forward_column_tile, row_lanes;
     int inverse_column_fused, forward_column_fused, row_fused;
+    int inverse_column_lanes, forward_column_lanes;   /* lanes per column workgroup */
 } shhd_info;
 
 typedef struct shhd_gpu shhd_gpu;

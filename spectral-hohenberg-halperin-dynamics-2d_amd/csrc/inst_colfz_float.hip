@@ -2,6 +2,6 @@
 #define SHHD_DEFINE_INSTANCES
 #include "instances.h"
 namespace shhd {
-#define SHHD_B(nt, len) SHHD_I_COLFZ(template, float, nt, len)
+#define SHHD_B(nt, v, len) SHHD_I_COLFZ(template, float, nt, v, len)
 SHHD_FOR_COL_FZ(SHHD_B)
 }  // namespace shhd

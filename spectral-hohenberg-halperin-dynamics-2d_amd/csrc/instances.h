@@ -12,11 +12,11 @@
 // X(NT, V, L), the lane shape the row plan picks for rows of L points.
 #define SHHD_FOR_ROW_FL(X) X(256, 6, 768) X(256, 6, 1536) X(512, 6, 3072) X(1024, 6, 6144)
 
-// Column kernels with fused transforms: X(NT, L), C = NT * 6 / L columns of
-// L points per workgroup (3/2-padded sides 64 .. 4096).
-#define SHHD_FOR_COL_FZ(X)                                                                            \
-    X(256, 96) X(64, 192) X(256, 192) X(64, 384) X(256, 384) X(256, 768) X(512, 768) X(256, 1536)    \
-    X(512, 1536) X(512, 3072) X(1024, 3072) X(1024, 6144)
+// Column kernels with fused transforms: X(NT, V, L), C = NT * V / L columns
+// of L points per workgroup (3/2-padded sides 64 .. 4096).
+#define SHHD_FOR_COL_FZ(X)                                                                             \
+    X(256, 6, 96) X(64, 6, 192) X(256, 6, 192) X(64, 6, 384) X(256, 6, 384) X(256, 6, 768)           \
+    X(512, 6, 768) X(256, 6, 1536) X(512, 6, 1536) X(512, 6, 3072) X(1024, 6, 3072) X(1024, 6, 6144)
 
 // order-parameter counts the row kernel is built for (fields held per lane)
 #define SHHD_FOR_MAXN(X, ...) X(__VA_ARGS__, 0) X(__VA_ARGS__, 1) X(__VA_ARGS__, 2) X(__VA_ARGS__, 4) X(__VA_ARGS__, 8)
@@ -43,18 +43,18 @@ namespace shhd {
 #define SHHD_I_ROWFL(EXT, T, nt, v, fl)                                                              \
     SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 0) SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 1)  \
     SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 2)
-// fused column kernels, one and two LDS buffers
-#define SHHD_I_COLFZ(EXT, T, nt, len)                                                                \
-    SHHD_I_K1(EXT, T, nt, 6, len, 1) SHHD_I_K1(EXT, T, nt, 6, len, 2) SHHD_I_K3(EXT, T, nt, 6, 4, len, 1) \
-    SHHD_I_K3(EXT, T, nt, 6, 4, len, 2)
+// fused column kernels, one and two LDS buffers (K3: two modes per radix-3 butterfly)
+#define SHHD_I_COLFZ(EXT, T, nt, v, len)                                                               \
+    SHHD_I_K1(EXT, T, nt, v, len, 1) SHHD_I_K1(EXT, T, nt, v, len, 2)                                 \
+    SHHD_I_K3(EXT, T, nt, v, (2 * (v) / 3), len, 1) SHHD_I_K3(EXT, T, nt, v, (2 * (v) / 3), len, 2)
 
 #define SHHD_EXTERN extern template
 #define SHHD_D_SHAPE_F64(nt, v) SHHD_I_SHAPE(SHHD_EXTERN, double, nt, v)
 #define SHHD_D_SHAPE_F32(nt, v) SHHD_I_SHAPE(SHHD_EXTERN, float, nt, v)
 #define SHHD_D_ROWFL_F64(nt, v, fl) SHHD_I_ROWFL(SHHD_EXTERN, double, nt, v, fl)
 #define SHHD_D_ROWFL_F32(nt, v, fl) SHHD_I_ROWFL(SHHD_EXTERN, float, nt, v, fl)
-#define SHHD_D_COLFZ_F64(nt, len) SHHD_I_COLFZ(SHHD_EXTERN, double, nt, len)
-#define SHHD_D_COLFZ_F32(nt, len) SHHD_I_COLFZ(SHHD_EXTERN, float, nt, len)
+#define SHHD_D_COLFZ_F64(nt, v, len) SHHD_I_COLFZ(SHHD_EXTERN, double, nt, v, len)
+#define SHHD_D_COLFZ_F32(nt, v, len) SHHD_I_COLFZ(SHHD_EXTERN, float, nt, v, len)
 #ifndef SHHD_DEFINE_INSTANCES
 SHHD_FOR_NT_V(SHHD_D_SHAPE_F64)
 SHHD_FOR_NT_V(SHHD_D_SHAPE_F32)

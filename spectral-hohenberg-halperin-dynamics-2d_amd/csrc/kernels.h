@@ -60,7 +60,7 @@ __device__ __forceinline__ int signed_gy(int gy, int my) { return (gy <= my / 2)
 // ---------------------------------------------------------------------------
 // K1
 // ---------------------------------------------------------------------------
-// FZL > 0: a build for columns of exactly FZL points, C = NT * 6 / FZL per
+// FZL > 0: a build for columns of exactly FZL points, C = NT * VPT / FZL per
 // workgroup, whose first transform pass loads the compact spectra straight
 // into its butterflies (no LDS staging) and whose passes alternate between
 // NBUF LDS buffers.
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
     const T scale = T(d.in_scale);
     const cx<T>* res = lds;   // the transformed columns, natural order
     if constexpr (FZL > 0) {
-        constexpr int CF = NT * 6 / FZL;
+        constexpr int CF = NT * VPT / FZL;
         constexpr int R0 = fz_radix(FZL, false, 0);
         using M0 = FzMap<NT, FZL, CF, R0>;
         const int tid = threadIdx.x;
@@ -267,12 +267,13 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
     F.clear();
 
     if constexpr (NBUF > 0) {
-        static_assert(FL > 0 && PPT == 6, "fused row transforms need a fixed length and six points per lane");
-        constexpr int RF = NT * 6 / FL;              // rows per workgroup
+        static_assert(FL > 0 && PPT % 3 == 0, "fused row transforms need a fixed length and radix-3 butterflies per lane");
+        constexpr int RF = NT * PPT / FL;              // rows per workgroup
         constexpr int M3 = FL / 3;
         constexpr int R0 = fz_radix(FL, false, 0);
         using MI = FzMap<NT, FL, RF, R0>;
-        static_assert(FzMap<NT, FL, RF, 3>::B == 2 && FzMap<NT, FL, RF, 3>::NB == 2 * NT, "two radix-3 butterflies per lane");
+        constexpr int B3 = PPT / 3;                 // radix-3 butterflies per lane
+        static_assert(FzMap<NT, FL, RF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
         FzBufs<T, NBUF> bf{lds, lds + lp_size(RF * FL), 0};
 
         // ---- inverse x transforms, two fields per complex FFT -------------
@@ -312,12 +313,12 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 }
             }
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
-            cx<T> w[2][3];
+            cx<T> w[B3][3];
             fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
             const int ra = d.in_role[s];
             const int rb = has_b ? d.in_role[s + 1] : -1;
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < B3; ++b)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
                     F.store(ra, 3 * b + p, w[b][p].re);
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         if (d.make_velocity) {
             const T r0 = T(d.rho0);
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
+            for (int i = 0; i < PPT; ++i) {
                 const int id = tid + NT * (i / 3);
                 if (F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + id / M3 < my) atomicOr(flag, 1);
             }
@@ -341,9 +342,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const OutItem ia = d.out[o];
             const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
             F.fence();
-            cx<T> v[2][3];
+            cx<T> v[B3][3];
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < B3; ++b)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
                     const T va = F.eval(d, ia, 3 * b + p);
@@ -578,7 +579,7 @@ __device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<dou
 // ---------------------------------------------------------------------------
 // K3
 // ---------------------------------------------------------------------------
-// FZL > 0: a build for columns of exactly FZL points, C = NT * 6 / FZL per
+// FZL > 0: a build for columns of exactly FZL points, C = NT * VPT / FZL per
 // workgroup, with Nay <= 2 FZL / 3.  The transform loads U straight into its
 // first butterflies and ends with the radix-3 pass in registers: lane b's
 // butterfly j (of a column s) then holds ky = j (output j) and ky = j - FZL/3
@@ -692,12 +693,13 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     cx<T>* acc_x = lds;
     cx<T>* acc_y = lds + nmodes;
     if constexpr (FZL > 0) {
-        constexpr int CF = NT * 6 / FZL;
+        constexpr int CF = NT * VPT / FZL;
         constexpr int M3 = FZL / 3;
         constexpr int R0 = fz_radix(FZL, false, 0);
         constexpr int NP = fz_npass(FZL, false);
-        static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 4, "radix-3 last pass, four modes per lane");
-        static_assert(FzMap<NT, FZL, CF, 3>::NB == 2 * NT, "two radix-3 butterflies per lane");
+        constexpr int B3 = VPT / 3;                 // radix-3 butterflies per lane
+        static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 2 * B3, "radix-3 last pass, two modes per butterfly");
+        static_assert(FzMap<NT, FZL, CF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
         using M0 = FzMap<NT, FZL, CF, R0>;
         FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
         for (int ci = 0; ci < g.ncontrib; ++ci) {
@@ -719,14 +721,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     v[b][q] = gx < hxa ? gload(src + col + (static_cast<size_t>(j + q * M0::M) << d.ulg)) : czero<T>();
             }
             fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
-            cx<T> w[2][3];
+            cx<T> w[B3][3];
             fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
             // an opaque copy of the lane index: the mode coordinates below are
             // recomputed per contribution instead of held live across the transform
             int lane = tid;
             asm volatile("" : "+v"(lane));
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
+            for (int b = 0; b < B3; ++b) {
                 const int id = lane + NT * b;
                 const int s = id / M3;
                 const int j = id - s * M3;
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         // with the transform buffers
         __syncthreads();
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < B3; ++b) {
             const int id = tid + NT * b;
             const int s = id / M3;
             const int j = id - s * M3;

@@ -154,9 +154,9 @@ using K2Fn = void (*)(const InstDev<T>*, BlockMap);
 template <typename T>
 using K3Fn = void (*)(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
 
-// fused column builds exist for (NT, L)
-bool col_fused_built(int NT, int L) {
-#define SHHD_CFZ(nt, len) if (NT == nt && L == len) return true;
+// fused column builds exist for (NT, V, L)
+bool col_fused_built(int NT, int V, int L) {
+#define SHHD_CFZ(nt, v, len) if (NT == nt && V == v && L == len) return true;
     SHHD_FOR_COL_FZ(SHHD_CFZ)
 #undef SHHD_CFZ
     return false;
@@ -165,9 +165,9 @@ bool col_fused_built(int NT, int L) {
 template <typename T>
 K1Fn<T> pick_k1(int NT, int V, int fz, int nbuf) {
     if (fz > 0) {
-#define SHHD_K1FZ(nt, len)                                                                 \
-    if (NT == nt && fz == len) return nbuf == 2 ? &k_inverse_columns<T, nt, 6, len, 2> \
-                                                : &k_inverse_columns<T, nt, 6, len, 1>;
+#define SHHD_K1FZ(nt, v, len)                                                                       \
+    if (NT == nt && V == v && fz == len) return nbuf == 2 ? &k_inverse_columns<T, nt, v, len, 2> \
+                                                          : &k_inverse_columns<T, nt, v, len, 1>;
         SHHD_FOR_COL_FZ(SHHD_K1FZ)
 #undef SHHD_K1FZ
         return nullptr;
@@ -181,10 +181,10 @@ K1Fn<T> pick_k1(int NT, int V, int fz, int nbuf) {
 template <typename T>
 K3Fn<T> pick_k3(int NT, int V, int acc, int fz, int nbuf) {
     if (fz > 0) {
-        if (acc != 4) return nullptr;
-#define SHHD_K3FZ(nt, len)                                                                    \
-    if (NT == nt && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, 6, 4, len, 2> \
-                                                : &k_forward_columns<T, nt, 6, 4, len, 1>;
+        if (acc != 2 * V / 3) return nullptr;
+#define SHHD_K3FZ(nt, v, len)                                                                                  \
+    if (NT == nt && V == v && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 2> \
+                                                          : &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 1>;
         SHHD_FOR_COL_FZ(SHHD_K3FZ)
 #undef SHHD_K3FZ
         return nullptr;
@@ -800,19 +800,24 @@ void apply_plans(Instance& h, const Tuning& t) {
     h.cols = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 32, t.c1);
     h.cols3 = plan_columns(d.my, d.nay, d.hxa, h.elem, nt_min, 16, t.c3);
     h.rows = plan_rows(d.mx, d.my, h.elem, std::max(nt_min, t.rnt));
-    if (t.row_transform == 1 || t.row_transform == 2) h.rows.fl = 0;
-    if (t.row_transform >= 3 && h.rows.fl > 0) {
+    const int rt = t.row_transform, ct = t.col_transform;
+    if (rt == 1 || rt == 2) h.rows.fl = 0;
+    if (rt >= 3 && h.rows.fl > 0) {
         const size_t one = static_cast<size_t>(lp_size(h.rows.R * d.mx)) * h.elem;
-        h.rows.nbuf = (t.row_transform == 4 && 2 * one <= kLdsMax) ? 2 : 1;
+        h.rows.nbuf = (rt == 4 && 2 * one <= kLdsMax) ? 2 : 1;
         h.rows.lds = h.rows.nbuf * one;
     }
-    // fused column builds: C * My = NT * 6 and (K3) every kept ky within the
+    // fused column builds: C * My = NT * V and (K3) every kept ky within the
     // first and last thirds of the column
-    const int fz_nb = t.col_transform == 3 ? 1 : (t.col_transform == 4 ? 2 : 0);
+    const int fz_nb = ct == 3 ? 1 : (ct == 4 ? 2 : 0);
     if (fz_nb > 0 && d.my % 3 == 0 && 3 * d.nay <= 2 * d.my) {
         for (ColPlan* p : {&h.cols, &h.cols3}) {
-            if (p->NT * 6 != p->C * d.my || !col_fused_built(p->NT, d.my) || (p == &h.cols3 && p->acc != 4)) continue;
+            const int v = 6;
+            if ((p->C * d.my) % v != 0 || !col_fused_built(p->C * d.my / v, v, d.my)) continue;
             const size_t one = static_cast<size_t>(lp_size(p->C * d.my)) * h.elem;
+            p->NT = p->C * d.my / v;
+            p->V = v;
+            p->acc = 2 * v / 3;
             p->fz = d.my;
             p->nbuf = (fz_nb == 2 && 2 * one <= kLdsMax) ? 2 : 1;
             p->lds = p->nbuf * one;
@@ -2298,6 +2303,8 @@ shhd_status shhd_gpu_get_info(const shhd_gpu* h, shhd_info* info) {
     info->inverse_column_fused = I->cols.fz > 0 ? I->cols.nbuf : 0;
     info->forward_column_fused = I->cols3.fz > 0 ? I->cols3.nbuf : 0;
     info->row_fused = I->rows.nbuf;
+    info->inverse_column_lanes = I->cols.NT;
+    info->forward_column_lanes = I->cols3.NT;
     return SHHD_OK;
 }
 

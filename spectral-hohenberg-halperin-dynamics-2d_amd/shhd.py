@@ -106,7 +106,8 @@ class ShhdInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_size_t), ("inverse_column_tile", ctypes.c_int),
                 ("forward_column_tile", ctypes.c_int), ("row_lanes", ctypes.c_int),
                 ("inverse_column_fused", ctypes.c_int), ("forward_column_fused", ctypes.c_int),
-                ("row_fused", ctypes.c_int)]
+                ("row_fused", ctypes.c_int), ("inverse_column_lanes", ctypes.c_int),
+                ("forward_column_lanes", ctypes.c_int)]
 
 
 @dataclass

@@ -118,3 +118,4 @@ def test_plane_blocks_give_identical_results(fp32):
             for f in range(cfg.num_fields):
                 assert np.array_equal(out[0][f], ref[0][f]), (ct, pb, f)
             assert np.array_equal(out[1], ref[1]), (ct, pb)
+
