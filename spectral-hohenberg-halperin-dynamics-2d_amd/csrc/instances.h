@@ -20,10 +20,14 @@
 
 // order-parameter counts the row kernel is built for (fields held per lane)
 #define SHHD_FOR_MAXN(X, ...) X(__VA_ARGS__, 0) X(__VA_ARGS__, 1) X(__VA_ARGS__, 2) X(__VA_ARGS__, 4) X(__VA_ARGS__, 8)
+// fixed field layouts of the fused row kernel, (layout, order parameters):
+// compressible N = 0, 1, 2; incompressible N = 0; quiescent N = 1
+#define SHHD_FOR_LAYOUT(X, ...) X(__VA_ARGS__, 1, 0) X(__VA_ARGS__, 1, 1) X(__VA_ARGS__, 1, 2) X(__VA_ARGS__, 2, 0) \
+    X(__VA_ARGS__, 3, 1)
 
 namespace shhd {
 
-// K1<T, NT, V, FZL, NBUF>, K3<T, NT, V, ACC, FZL, NBUF>, K2<T, NT, V, MAXN, FL, NBUF>
+// K1<T, NT, V, FZL, NBUF>, K3<T, NT, V, ACC, FZL, NBUF>, K2<T, NT, V, MAXN, FL, NBUF, LAYOUT>
 #define SHHD_K1_SIG(T) (const InstDev<T>*, const StagePtrs<T>*, BlockMap)
 #define SHHD_K3_SIG(T) (const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap)
 #define SHHD_K2_SIG(T) (const InstDev<T>*, BlockMap)
@@ -32,7 +36,8 @@ namespace shhd {
 // EXT is `extern template` for the declarations, `template` in the unit that builds them
 #define SHHD_I_K1(EXT, T, nt, v, fz, nb) EXT __global__ void k_inverse_columns<T, nt, v, fz, nb> SHHD_K1_SIG(T);
 #define SHHD_I_K3(EXT, T, nt, v, a, fz, nb) EXT __global__ void k_forward_columns<T, nt, v, a, fz, nb> SHHD_K3_SIG(T);
-#define SHHD_I_K2(EXT, T, nt, v, fl, nb, mn) EXT __global__ void k_rows<T, nt, v, mn, fl, nb> SHHD_K2_SIG(T);
+#define SHHD_I_K2(EXT, T, nt, v, fl, nb, mn) EXT __global__ void k_rows<T, nt, v, mn, fl, nb, 0> SHHD_K2_SIG(T);
+#define SHHD_I_K2L(EXT, T, nt, v, fl, nb, lay, mn) EXT __global__ void k_rows<T, nt, v, mn, fl, nb, lay> SHHD_K2_SIG(T);
 #define SHHD_I_KP(EXT, T, nt, v) EXT __global__ void k_rows_physical<T, nt, v> SHHD_KP_SIG(T);
 
 // staged column and row kernels of every lane shape, any length
@@ -40,9 +45,11 @@ namespace shhd {
     SHHD_I_K1(EXT, T, nt, v, 0, 1) SHHD_I_K3(EXT, T, nt, v, 4, 0, 1) SHHD_I_K3(EXT, T, nt, v, v, 0, 1) \
     SHHD_I_KP(EXT, T, nt, v) SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, 0, 0)
 // fixed-length row kernels: staged, fused with one and with two LDS buffers
+// (any field order, and the fixed layouts)
 #define SHHD_I_ROWFL(EXT, T, nt, v, fl)                                                              \
     SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 0) SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 1)  \
-    SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 2)
+    SHHD_FOR_MAXN(SHHD_I_K2, EXT, T, nt, v, fl, 2) SHHD_FOR_LAYOUT(SHHD_I_K2L, EXT, T, nt, v, fl, 1) \
+    SHHD_FOR_LAYOUT(SHHD_I_K2L, EXT, T, nt, v, fl, 2)
 // fused column kernels, one and two LDS buffers (K3: two modes per radix-3 butterfly)
 #define SHHD_I_COLFZ(EXT, T, nt, v, len)                                                               \
     SHHD_I_K1(EXT, T, nt, v, len, 1) SHHD_I_K1(EXT, T, nt, v, len, 2)                                 \

@@ -186,6 +186,14 @@ struct RowFields {
 #pragma unroll
         for (int a = 0; a < MAXN; ++a) psi[a][i] = role == ROLE_PSI0 + a ? v : psi[a][i];
     }
+    // the same store for a role known at compile time
+    template <int ROLE>
+    __device__ __forceinline__ void store_ct(int i, T v) {
+        if constexpr (ROLE == ROLE_RHO) rho[i] = v;
+        else if constexpr (ROLE == ROLE_JX) jx[i] = v;
+        else if constexpr (ROLE == ROLE_JY) jy[i] = v;
+        else psi[ROLE - ROLE_PSI0][i] = v;
+    }
     __device__ __forceinline__ T psi_of(int alpha, int i) const {
         T v = T(0);
 #pragma unroll
@@ -231,6 +239,18 @@ __device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPl
     else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
 }
 
+// Inverse-transformed fields in a fixed order, known at compile time:
+// LAYOUT 1: psi_0 .. psi_{MAXN-1}, rho, j_x, j_y; 2: psi_0 .. psi_{MAXN-1},
+// j_x, j_y; 3: psi_0 .. psi_{MAXN-1}; 0: any order (roles read at run time).
+constexpr int layout_nin(int layout, int maxn) {
+    return layout == 1 ? maxn + 3 : (layout == 2 ? maxn + 2 : maxn);
+}
+constexpr int layout_role(int layout, int maxn, int p) {
+    if (p < maxn) return ROLE_PSI0 + p;
+    if (layout == 1) return p == maxn ? ROLE_RHO : (p == maxn + 1 ? ROLE_JX : ROLE_JY);
+    return p == maxn ? ROLE_JX : ROLE_JY;
+}
+
 // NBUF > 0 (FL > 0 and PPT = 6 only): the inverse transform runs with the
 // radix-3 pass last and leaves each lane the six points j, j + FL/3, j + 2FL/3
 // of its two radix-3 butterflies; the products at those points feed the
@@ -238,7 +258,7 @@ __device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPl
 // inverse's first pass loads T straight into its butterflies, and the passes
 // alternate between NBUF LDS buffers.  NBUF = 0: every transform staged
 // through one LDS buffer.
-template <typename T, int NT, int PPT, int MAXN, int FL, int NBUF>
+template <typename T, int NT, int PPT, int MAXN, int FL, int NBUF, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
@@ -277,8 +297,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         FzBufs<T, NBUF> bf{lds, lds + lp_size(RF * FL), 0};
 
         // ---- inverse x transforms, two fields per complex FFT -------------
-        for (int s = 0; s < d.n_in; s += 2) {
-            const bool has_b = s + 1 < d.n_in;
+        // fields s and s + 1 (if has_b) of T into w: lane b's butterfly holds
+        // the points j, j + FL/3, j + 2FL/3 of its row
+        auto transform_pair = [&](int s, bool has_b, cx<T> (&w)[B3][3]) {
             const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
             const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
             cx<T> v[MI::B][R0];
@@ -313,17 +334,41 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 }
             }
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
-            cx<T> w[B3][3];
             fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
-            const int ra = d.in_role[s];
-            const int rb = has_b ? d.in_role[s + 1] : -1;
+        };
+        if constexpr (LAYOUT > 0) {
+            // roles known at compile time: every value goes straight to its array
+            constexpr int NIN = layout_nin(LAYOUT, MAXN);
+            auto pair_ct = [&]<int S>() {
+                constexpr bool HB = S + 1 < NIN;
+                cx<T> w[B3][3];
+                transform_pair(S, HB, w);
 #pragma unroll
-            for (int b = 0; b < B3; ++b)
+                for (int b = 0; b < B3; ++b)
 #pragma unroll
-                for (int p = 0; p < 3; ++p) {
-                    F.store(ra, 3 * b + p, w[b][p].re);
-                    if (rb >= 0) F.store(rb, 3 * b + p, w[b][p].im);
-                }
+                    for (int p = 0; p < 3; ++p) {
+                        F.template store_ct<layout_role(LAYOUT, MAXN, S)>(3 * b + p, w[b][p].re);
+                        if constexpr (HB) F.template store_ct<layout_role(LAYOUT, MAXN, S + 1)>(3 * b + p, w[b][p].im);
+                    }
+            };
+            [&]<int... P>(std::integer_sequence<int, P...>) {
+                (pair_ct.template operator()<2 * P>(), ...);
+            }(std::make_integer_sequence<int, (NIN + 1) / 2>{});
+        } else {
+            for (int s = 0; s < d.n_in; s += 2) {
+                const bool has_b = s + 1 < d.n_in;
+                cx<T> w[B3][3];
+                transform_pair(s, has_b, w);
+                const int ra = d.in_role[s];
+                const int rb = has_b ? d.in_role[s + 1] : -1;
+#pragma unroll
+                for (int b = 0; b < B3; ++b)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) {
+                        F.store(ra, 3 * b + p, w[b][p].re);
+                        if (rb >= 0) F.store(rb, 3 * b + p, w[b][p].im);
+                    }
+            }
         }
 
         // ---- velocity (src/fcalculator_workspace.cc:122-133) -----------------

@@ -58,6 +58,7 @@ struct RowPlan {
     int R = 1, NT = 256, V = 6, nblocks = 1, maxn = 0;
     int fl = 0;        // row length of a fixed-length row-kernel build, 0 for the any-length one
     int nbuf = 0;      // fixed-length build: 0 staged, 1 or 2 fused with that many LDS buffers
+    int layout = 0;    // fused build: fixed order of the inverse-transformed fields (0: any)
     size_t lds = 0;
 };
 
@@ -205,20 +206,39 @@ int row_fixed_length(int mx, int NT, int V) {
     return 0;
 }
 
+// a fused row build with a fixed field layout exists
+bool row_layout_built(int NT, int V, int maxn, int fl, int nbuf, int layout) {
+#define SHHD_RLN(nt, v, len, nb, lay, mn) \
+    if (NT == nt && V == v && fl == len && nbuf == nb && layout == lay && maxn == mn) return true;
+#define SHHD_RL(nt, v, len) SHHD_FOR_LAYOUT(SHHD_RLN, nt, v, len, 1) SHHD_FOR_LAYOUT(SHHD_RLN, nt, v, len, 2)
+    SHHD_FOR_ROW_FL(SHHD_RL)
+#undef SHHD_RL
+#undef SHHD_RLN
+    return false;
+}
+
 template <typename T>
-K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf) {
+K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf, int layout) {
     const int n = maxn <= 0 ? 0 : (maxn == 1 ? 1 : (maxn == 2 ? 2 : (maxn <= 4 ? 4 : 8)));
+    if (fl > 0 && nbuf > 0 && layout > 0) {
+#define SHHD_K2LN(nt, v, len, nb, lay, mn) \
+    if (NT == nt && V == v && fl == len && nbuf == nb && layout == lay && n == mn) return &k_rows<T, nt, v, mn, len, nb, lay>;
+#define SHHD_K2L(nt, v, len) SHHD_FOR_LAYOUT(SHHD_K2LN, nt, v, len, 1) SHHD_FOR_LAYOUT(SHHD_K2LN, nt, v, len, 2)
+        SHHD_FOR_ROW_FL(SHHD_K2L)
+#undef SHHD_K2L
+#undef SHHD_K2LN
+    }
     if (fl > 0) {
 #define SHHD_K2FN(nt, v, len, mn)                                                  \
     if (NT == nt && V == v && n == mn && fl == len)                                \
-        return nbuf == 2 ? &k_rows<T, nt, v, mn, len, 2>                           \
-                         : (nbuf == 1 ? &k_rows<T, nt, v, mn, len, 1> : &k_rows<T, nt, v, mn, len, 0>);
+        return nbuf == 2 ? &k_rows<T, nt, v, mn, len, 2, 0>                        \
+                         : (nbuf == 1 ? &k_rows<T, nt, v, mn, len, 1, 0> : &k_rows<T, nt, v, mn, len, 0, 0>);
 #define SHHD_K2F(nt, v, len) SHHD_FOR_MAXN(SHHD_K2FN, nt, v, len)
         SHHD_FOR_ROW_FL(SHHD_K2F)
 #undef SHHD_K2F
 #undef SHHD_K2FN
     }
-#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0>;
+#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0, 0>;
 #define SHHD_K2(nt, v) SHHD_K2N(nt, v, 0) SHHD_K2N(nt, v, 1) SHHD_K2N(nt, v, 2) SHHD_K2N(nt, v, 4) SHHD_K2N(nt, v, 8)
     SHHD_FOR_NT_V(SHHD_K2)
 #undef SHHD_K2
@@ -832,6 +852,17 @@ void apply_plans(Instance& h, const Tuning& t) {
     for (int i = 0; i < d.n_in; ++i)
         if (d.in_role[i] >= ROLE_PSI0) npsi_roles = std::max(npsi_roles, d.in_role[i] - ROLE_PSI0 + 1);
     h.rows.maxn = npsi_roles == 0 ? 0 : (npsi_roles == 1 ? 1 : (npsi_roles == 2 ? 2 : (npsi_roles <= 4 ? 4 : 8)));
+    // a fused row build for this exact field order, where one is built
+    h.rows.layout = 0;
+    if (h.rows.nbuf > 0 && h.rows.maxn == npsi_roles) {
+        for (int lay = 1; lay <= 3 && h.rows.layout == 0; ++lay) {
+            if (layout_nin(lay, npsi_roles) != d.n_in || d.n_in == 0) continue;
+            bool match = true;
+            for (int p = 0; p < d.n_in; ++p) match = match && d.in_role[p] == layout_role(lay, npsi_roles, p);
+            if (match && row_layout_built(h.rows.NT, h.rows.V, h.rows.maxn, h.rows.fl, h.rows.nbuf, lay))
+                h.rows.layout = lay;
+        }
+    }
 }
 
 // SRK3 with noise in the FP32 form keeps the step's draws A and B for
@@ -1017,11 +1048,11 @@ shhd_status download_state(const Instance* h, double* out, size_t n, hipStream_t
 struct ClassKey {
     bool fp32;
     int stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl;
-    int cfz, cnb, c3fz, c3nb, rnb;   // fused builds (length, LDS buffers)
+    int cfz, cnb, c3fz, c3nb, rnb, rlay;   // fused builds (length, LDS buffers, field layout)
     bool operator<(const ClassKey& o) const {
-        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl, cfz, cnb, c3fz, c3nb, rnb) <
+        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl, cfz, cnb, c3fz, c3nb, rnb, rlay) <
                std::tie(o.fp32, o.stages, o.cNT, o.cV, o.c3NT, o.c3V, o.acc, o.rNT, o.rV, o.maxn, o.rfl, o.cfz, o.cnb,
-                        o.c3fz, o.c3nb, o.rnb);
+                        o.c3fz, o.c3nb, o.rnb, o.rlay);
     }
 };
 
@@ -1703,7 +1734,7 @@ struct Engine {
         const ClassKey& K = lc.key;
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV, K.cfz, K.cnb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
@@ -1721,7 +1752,7 @@ struct Engine {
             if (!h) continue;
             ClassKey k{h->fp32,      h->stages,     h->cols.NT,     h->cols.V,       h->cols3.NT, h->cols3.V,
                        h->cols3.acc, h->rows.NT,    h->rows.V,      h->rows.maxn,    h->rows.fl,  h->cols.fz,
-                       h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf};
+                       h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf, h->rows.layout};
             groups[k].push_back(s);
         }
         for (auto& g : groups) {
@@ -1782,7 +1813,7 @@ struct Engine {
             if (st != SHHD_OK) return st;
         }
         if (lc.nblk[1] > 0) {
-            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb);
+            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay);
             st = timed(1, capturing, [&] {
                 hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], stream, inst, lc.map[1]);
             });
