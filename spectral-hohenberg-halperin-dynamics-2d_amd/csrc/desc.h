@@ -49,6 +49,8 @@ struct ContribCoef {
     double a[2][4];
     double b[2][2];
     double f[3][3];
+    // nonzero entries: a[r][t] bit 4r + t, b[r][t] bit 8 + 2r + t, f[t][u] bit 12 + 3t + u
+    unsigned nz;
 };
 
 struct Group {

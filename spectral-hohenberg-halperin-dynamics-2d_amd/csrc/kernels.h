@@ -708,27 +708,43 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     const bool recording = psi_group ? fx_slot != nullptr : pi_slot != nullptr;
     // one transformed mode v of column gx, signed ky_i, compact slot cidx
     // into accumulator slot i (and the step flux when recording)
+    // Only the coefficients a product has (cf.nz, a uniform mask) are
+    // evaluated: the branches are scalar and skip the vector work of the
+    // zero terms.
     auto accumulate = [&](const ContribCoef& cf, int i, cx<T> v, int gx, int ky_i, int cidx) {
+        const unsigned nz = cf.nz;
         const T kx = T(d.dkx) * T(gx);
         const T ky = T(d.dky) * T(ky_i);
-        const T kxx = kx * kx, kyy = ky * ky, kxy = kx * ky;
         const cx<T> mv = mul_mi(v);   // -i v
-        {
-            const T a = T(cf.a[0][0]) + T(cf.a[0][1]) * kxx + T(cf.a[0][2]) * kyy + T(cf.a[0][3]) * kxy;
-            const T b = T(cf.b[0][0]) * kx + T(cf.b[0][1]) * ky;
-            acc0[i] = acc0[i] + a * v + b * mv;
-        }
+        // a[r][0] + a[r][1] kx^2 + a[r][2] ky^2 + a[r][3] kx ky and b[r][0] kx + b[r][1] ky
+        auto poly = [&](int r) {
+            const unsigned m = nz >> (4 * r);
+            T p = T(0);
+            if (m & 1u) p = p + T(cf.a[r][0]);
+            if (m & 2u) p = p + T(cf.a[r][1]) * (kx * kx);
+            if (m & 4u) p = p + T(cf.a[r][2]) * (ky * ky);
+            if (m & 8u) p = p + T(cf.a[r][3]) * (kx * ky);
+            return p;
+        };
+        auto lin = [&](int r) {
+            const unsigned m = nz >> (8 + 2 * r);
+            T p = T(0);
+            if (m & 1u) p = p + T(cf.b[r][0]) * kx;
+            if (m & 2u) p = p + T(cf.b[r][1]) * ky;
+            return p;
+        };
+        if (nz & 0x00Fu) acc0[i] = acc0[i] + poly(0) * v;
+        if (nz & 0x300u) acc0[i] = acc0[i] + lin(0) * mv;
         if (!psi_group) {
-            const T a = T(cf.a[1][0]) + T(cf.a[1][1]) * kxx + T(cf.a[1][2]) * kyy + T(cf.a[1][3]) * kxy;
-            const T b = T(cf.b[1][0]) * kx + T(cf.b[1][1]) * ky;
-            acc1[i] = acc1[i] + a * v + b * mv;
+            if (nz & 0x0F0u) acc1[i] = acc1[i] + poly(1) * v;
+            if (nz & 0xC00u) acc1[i] = acc1[i] + lin(1) * mv;
         }
         if (recording) {
             const size_t at = static_cast<size_t>(gx) * nay + cidx;
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 cx<T>* dst = psi_group ? (t == 0 ? fx_slot : (t == 1 ? fy_slot : nullptr)) : pi_slot + t * fe;
-                if (dst == nullptr) continue;
+                if (dst == nullptr || ((nz >> (12 + 3 * t)) & 7u) == 0) continue;
                 const cx<T> fv = T(cf.f[t][0]) * v + (T(cf.f[t][1]) * kx + T(cf.f[t][2]) * ky) * mv;
                 dst[at] = dst[at] + fw * fv;
             }

@@ -458,6 +458,16 @@ ContribCoef contrib_coef(const Desc& d, OutItem it) {
         case OUT_PXY: c.b[0][1] = 1.0; c.b[1][0] = 1.0; c.f[1][0] = 1.0; break;
         default: c.b[1][1] = 1.0; c.f[2][0] = 1.0; break;   // OUT_PYY
     }
+    c.nz = 0;
+    for (int r = 0; r < 2; ++r)
+        for (int t = 0; t < 4; ++t)
+            if (c.a[r][t] != 0.0) c.nz |= 1u << (4 * r + t);
+    for (int r = 0; r < 2; ++r)
+        for (int t = 0; t < 2; ++t)
+            if (c.b[r][t] != 0.0) c.nz |= 1u << (8 + 2 * r + t);
+    for (int t = 0; t < 3; ++t)
+        for (int u = 0; u < 3; ++u)
+            if (c.f[t][u] != 0.0) c.nz |= 1u << (12 + 3 * t + u);
     return c;
 }
 
