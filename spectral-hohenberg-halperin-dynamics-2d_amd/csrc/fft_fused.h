@@ -65,8 +65,18 @@ struct FzBufs {
     }
 };
 
+// LDS slot of element i of a buffer in the padded (lp) or the plain layout
+template <bool PAD>
+__device__ __forceinline__ int fz_slot(int i) { return PAD ? lp(i) : i; }
+
+// Layout of the buffer pass P writes: plain after a radix-3 first pass (its
+// stride-3 writes are conflict-free unpadded and conflict 2-way padded; the
+// next pass reads runs of consecutive elements, conflict-free either way),
+// padded after every other pass (stride 8, 4, 2 writes).
+constexpr bool fz_pad_after(int L, bool r3_first, int P) { return !(P == 0 && fz_radix(L, r3_first, 0) == 3); }
+
 // Stockham outputs of the butterflies in v to LDS.
-template <typename T, int NT, int L, int NSEQ, int R, int NS>
+template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
 __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R], int tid) {
     using Mp = FzMap<NT, L, NSEQ, R>;
 #pragma unroll
@@ -77,19 +87,19 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
         const int j = id - s * Mp::M;
         const int k = j % NS;
         const int o = s * L + (j - k) * R + k;
-        if constexpr (NS % 16 == 0) {
+        if constexpr (PAD && NS % kPad == 0) {
             const int o0 = lp(o);
 #pragma unroll
-            for (int p = 0; p < R; ++p) out[o0 + p * (NS + NS / 16)] = v[b][p];
+            for (int p = 0; p < R; ++p) out[o0 + p * (NS + NS / kPad)] = v[b][p];
         } else {
 #pragma unroll
-            for (int p = 0; p < R; ++p) out[lp(o + p * NS)] = v[b][p];
+            for (int p = 0; p < R; ++p) out[fz_slot<PAD>(o + p * NS)] = v[b][p];
         }
     }
 }
 
 // Butterfly inputs from LDS, twiddled, transformed in registers.
-template <typename T, int NT, int L, int NSEQ, int R, int NS>
+template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
 __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R],
                                         const cx<T>* __restrict__ tw, int sign, int tid) {
     using Mp = FzMap<NT, L, NSEQ, R>;
@@ -101,13 +111,13 @@ __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)
         const int s = id / Mp::M;
         const int j = id - s * Mp::M;
         const int a = s * L + j;
-        if constexpr (Mp::M % 16 == 0) {
+        if constexpr (PAD && Mp::M % kPad == 0) {
             const int a0 = lp(a);
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[b][q] = in[a0 + q * (Mp::M + Mp::M / 16)];
+            for (int q = 0; q < R; ++q) v[b][q] = in[a0 + q * (Mp::M + Mp::M / kPad)];
         } else {
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[b][q] = in[lp(a + q * Mp::M)];
+            for (int q = 0; q < R; ++q) v[b][q] = in[fz_slot<PAD>(a + q * Mp::M)];
         }
     }
 #pragma unroll
@@ -138,7 +148,7 @@ __device__ __forceinline__ void fz_first(FzBufs<T, NBUF>& bf,
     for (int b = 0; b < Mp::B; ++b)
         if (Mp::valid(tid + NT * b)) dft_small<T, R>(v[b], sign);
     bf.before_write();
-    fz_write<T, NT, L, NSEQ, R, 1>(bf.wr(), v, tid);
+    fz_write<T, NT, L, NSEQ, R, 1, fz_pad_after(L, R3F, 0)>(bf.wr(), v, tid);
     bf.flip();
 }
 
@@ -149,9 +159,9 @@ __device__ __forceinline__ void fz_pass(FzBufs<T, NBUF>& bf, const cx<T>* __rest
     constexpr int NS = fz_ns(L, R3F, P);
     cx<T> v[FzMap<NT, L, NSEQ, R>::B][R];
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, R, NS>(bf.rd(), v, tw, sign, tid);
+    fz_read<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P - 1)>(bf.rd(), v, tw, sign, tid);
     bf.before_write();
-    fz_write<T, NT, L, NSEQ, R, NS>(bf.wr(), v, tid);
+    fz_write<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P)>(bf.wr(), v, tid);
     bf.flip();
 }
 
@@ -180,7 +190,8 @@ __device__ __forceinline__ void fz_rest_to_regs(
     asm volatile("" : "+v"(tid));
     fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 2>{});
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, fz_radix(L, R3F, NP - 1), fz_ns(L, R3F, NP - 1)>(bf.rd(), v, tw, sign, tid);
+    fz_read<T, NT, L, NSEQ, fz_radix(L, R3F, NP - 1), fz_ns(L, R3F, NP - 1), fz_pad_after(L, R3F, NP - 2)>(
+        bf.rd(), v, tw, sign, tid);
 }
 
 }  // namespace shhd

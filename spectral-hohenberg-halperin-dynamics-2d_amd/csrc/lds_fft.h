@@ -22,11 +22,17 @@ namespace shhd {
 
 constexpr int kMaxPasses = 16;
 
-// LDS slot of FFT element i: one pad slot after every 16 elements, so the
-// stride-R writes of the early Stockham passes spread over the banks.
-__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+// LDS slot of FFT element i: one pad slot after every 16 elements.  A 16-byte
+// LDS access of a wave is conflict-free when each 8 consecutive lanes hit 8
+// distinct 16-byte slots mod 8 (measured: 6 cycles per wave access; 10 for the
+// stride-8 writes of a first radix-8 pass under this padding, 48 unpadded).
+// A pad every 8 elements would make those conflict-free too, but the larger
+// buffers cost resident workgroups where LDS limits them (measured slower).
+constexpr int kPadShift = 4;
+constexpr int kPad = 1 << kPadShift;
+__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> kPadShift); }
 // LDS values needed for n padded elements
-__host__ __device__ __forceinline__ int lp_size(int n) { return n + (n >> 4) + 1; }
+__host__ __device__ __forceinline__ int lp_size(int n) { return n + (n >> kPadShift) + 1; }
 
 // One Stockham pass: radix R applied to sub-transforms of length ns (a power of
 // two: radix-3 passes run last), m = L/R butterflies per sequence, twiddle
@@ -221,10 +227,10 @@ __device__ __forceinline__ void fft_pass_ct(cx<T>* buf, int nseq, const cx<T>* _
             jj[b] = j;
             sl[b] = s * L;
             const int a = s * L + j;
-            if constexpr (M % 16 == 0) {
+            if constexpr (M % kPad == 0) {
                 const int a0 = lp(a);
 #pragma unroll
-                for (int q = 0; q < R; ++q) v[b][q] = buf[a0 + q * (M + M / 16)];
+                for (int q = 0; q < R; ++q) v[b][q] = buf[a0 + q * (M + M / kPad)];
             } else {
 #pragma unroll
                 for (int q = 0; q < R; ++q) v[b][q] = buf[lp(a + q * M)];
@@ -243,10 +249,10 @@ __device__ __forceinline__ void fft_pass_ct(cx<T>* buf, int nseq, const cx<T>* _
             }
             dft_small<T, R>(v[b], sign);
             const int out = sl[b] + ((j >> LNS) << LNS) * R + k;
-            if constexpr (NS % 16 == 0) {
+            if constexpr (NS % kPad == 0) {
                 const int o0 = lp(out);
 #pragma unroll
-                for (int p = 0; p < R; ++p) buf[o0 + p * (NS + NS / 16)] = v[b][p];
+                for (int p = 0; p < R; ++p) buf[o0 + p * (NS + NS / kPad)] = v[b][p];
             } else {
 #pragma unroll
                 for (int p = 0; p < R; ++p) buf[lp(out + p * NS)] = v[b][p];
