@@ -56,6 +56,17 @@ __device__ __forceinline__ void gstore(cx<T>* p, cx<T> z) {
     v.y = z.im;
     *(__attribute__((address_space(1))) V*)(p) = v;
 }
+// Element i of an array whose base is uniform across the wave: a scalar base
+// and a 32-bit lane byte offset (i * sizeof, which must stay below 4 GiB: every
+// array addressed this way is one field or plane)
+template <typename T>
+__device__ __forceinline__ cx<T> gload(const cx<T>* base, uint32_t i) {
+    return gload(reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(cx<T>))));
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cx<T>* base, uint32_t i, cx<T> z) {
+    gstore(reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(cx<T>))), z);
+}
 #endif
 
 // ---------------------------------------------------------------------------

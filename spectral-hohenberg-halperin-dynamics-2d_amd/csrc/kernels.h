@@ -106,7 +106,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
                 v[b][q] = czero<T>();
                 if (gx < d.hxa && ky < half && -ky < half) {
                     const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v[b][q] = scale * gload(src + static_cast<size_t>(gx) * d.nay + cidx);
+                    v[b][q] = scale * gload(src, static_cast<uint32_t>(gx * d.nay + cidx));
                 }
             }
         }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
                 const int ky = signed_gy(y, L);
                 if (gx < d.hxa && ky < half && -ky < half) {
                     const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v = scale * gload(src + static_cast<size_t>(gx) * d.nay + cidx);
+                    v = scale * gload(src, static_cast<uint32_t>(gx * d.nay + cidx));
                 }
                 lds[lp(c * L + y)] = v;
             }
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         const int y = idx >> lc;
         const int c = idx - (y << lc);
         const int gx = gx0 + c;
-        if (gx < d.hxa) gstore(dst + static_cast<size_t>(y) * d.hxa + gx, res[lp(c * L + y)]);
+        if (gx < d.hxa) gstore(dst, static_cast<uint32_t>(y * d.hxa + gx), res[lp(c * L + y)]);
     }
 }
 
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 const int r = id / MI::M;
                 const int j = id - r * MI::M;
                 const int y = y0 + r;
-                const size_t row = static_cast<size_t>(y < my ? y : 0) * hxa;
+                const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
 #pragma unroll
                 for (int q = 0; q < R0; ++q) {
                     // one load per field: kx = k (k < Nax/2) or the mirror
@@ -324,8 +324,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     const int kk = mirror ? FL - k : k;
                     cx<T> z = czero<T>();
                     if (y < my && (k < hxa || mirror)) {
-                        cx<T> a = gload(ta + row + kk);
-                        cx<T> bb = has_b ? gload(tb + row + kk) : czero<T>();
+                        cx<T> a = gload(ta, row + kk);
+                        cx<T> bb = has_b ? gload(tb, row + kk) : czero<T>();
                         if (k == 0) { a.im = T(0); bb.im = T(0); }
                         if (mirror) { a.im = -a.im; bb.im = -bb.im; }   // conj(A) + i conj(B)
                         z = mk(a.re - bb.im, a.im + bb.re);             // A + iB
@@ -415,9 +415,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 if (y < my) {
                     const cx<T> zk = res[lp(r * FL + k)];
                     const cx<T> zm = res[lp(r * FL + (k == 0 ? 0 : FL - k))];
-                    const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);
-                    gstore(ua + at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
-                    if (has_b) gstore(ub + at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
+                    const uint32_t at = ((static_cast<uint32_t>(kb) * my + y) << d.ulg) + (k & umask);
+                    gstore(ua, at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
+                    if (has_b) gstore(ub, at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
                 }
             }
         }
@@ -435,16 +435,16 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const int y = y0 + r;
             cx<T> z = czero<T>();
             if (y < my) {
-                const size_t row = static_cast<size_t>(y) * hxa;
+                const uint32_t row = static_cast<uint32_t>(y * hxa);
                 if (k < hxa) {
-                    cx<T> a = gload(ta + row + k);
-                    cx<T> b = has_b ? gload(tb + row + k) : czero<T>();
+                    cx<T> a = gload(ta, row + k);
+                    cx<T> b = has_b ? gload(tb, row + k) : czero<T>();
                     if (k == 0) { a.im = T(0); b.im = T(0); }
                     z = mk(a.re - b.im, a.im + b.re);               // A + iB
                 } else if (k > L - hxa) {
                     const int kk = L - k;
-                    const cx<T> a = gload(ta + row + kk);
-                    const cx<T> b = has_b ? gload(tb + row + kk) : czero<T>();
+                    const cx<T> a = gload(ta, row + kk);
+                    const cx<T> b = has_b ? gload(tb, row + kk) : czero<T>();
                     z = mk(a.re + b.im, b.re - a.im);               // conj(A) + i conj(B)
                 }
             }
@@ -507,9 +507,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             if (y < my) {
                 const cx<T> zk = lds[lp(r * L + k)];
                 const cx<T> zm = lds[lp(r * L + (k == 0 ? 0 : L - k))];
-                const size_t at = ((static_cast<size_t>(kb) * my + y) << d.ulg) + (k & umask);   // kx-blocked U
-                gstore(ua + at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
-                if (has_b) gstore(ub + at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
+                const uint32_t at = ((static_cast<uint32_t>(kb) * my + y) << d.ulg) + (k & umask);   // kx-blocked U
+                gstore(ua, at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
+                if (has_b) gstore(ub, at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
             }
         }
         __syncthreads();
@@ -548,16 +548,16 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
             const int y = y0 + r;
             cx<T> z = czero<T>();
             if (y < my) {
-                const size_t row = static_cast<size_t>(y) * hxa;
+                const uint32_t row = static_cast<uint32_t>(y * hxa);
                 if (k < hxa) {
-                    cx<T> a = gload(ta + row + k);
-                    cx<T> b = has_b ? gload(tb + row + k) : czero<T>();
+                    cx<T> a = gload(ta, row + k);
+                    cx<T> b = has_b ? gload(tb, row + k) : czero<T>();
                     if (k == 0) { a.im = T(0); b.im = T(0); }
                     z = mk(a.re - b.im, a.im + b.re);
                 } else if (k > L - hxa) {
                     const int kk = L - k;
-                    const cx<T> a = gload(ta + row + kk);
-                    const cx<T> b = has_b ? gload(tb + row + kk) : czero<T>();
+                    const cx<T> a = gload(ta, row + kk);
+                    const cx<T> b = has_b ? gload(tb, row + kk) : czero<T>();
                     z = mk(a.re + b.im, b.re - a.im);
                 }
             }
@@ -669,6 +669,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     // Every mode is read and written by the one lane that owns it in all the
     // loops below (m = tid + NT * i), so the accumulation needs no atomics.
     const size_t fe = static_cast<size_t>(hxa) * nay;
+    const uint32_t fe32 = static_cast<uint32_t>(fe);
     cx<T>* fx_slot = nullptr;   // psi group: psi flux x; momentum group: rho flux x
     cx<T>* fy_slot = nullptr;
     cx<T>* pi_slot = nullptr;   // Pi xx, then Pi xy and Pi yy at +fe, +2fe
@@ -776,10 +777,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 const int s = id / M0::M;
                 const int j = id - s * M0::M;
                 const int gx = gx0 + s;
-                const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
+                const uint32_t col = (static_cast<uint32_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
 #pragma unroll
                 for (int q = 0; q < R0; ++q)   // kx-blocked U: consecutive lanes, consecutive rows
-                    v[b][q] = gx < hxa ? gload(src + col + (static_cast<size_t>(j + q * M0::M) << d.ulg)) : czero<T>();
+                    v[b][q] = gx < hxa ? gload(src, col + (static_cast<uint32_t>(j + q * M0::M) << d.ulg)) : czero<T>();
             }
             fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
             cx<T> w[B3][3];
@@ -825,9 +826,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(u) * plane;
             for (int c = 0; c < C; ++c) {   // kx-blocked U: a column's rows 2^ulg apart
                 const int gx = gx0 + c;
-                const size_t col = (static_cast<size_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
+                const uint32_t col = (static_cast<uint32_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
                 for (int y = tid; y < L; y += NT)
-                    lds[lp(c * L + y)] = gx < hxa ? gload(src + col + (static_cast<size_t>(y) << d.ulg)) : czero<T>();
+                    lds[lp(c * L + y)] = gx < hxa ? gload(src, col + (static_cast<uint32_t>(y) << d.ulg)) : czero<T>();
             }
             __syncthreads();
             col_fft<T, NT, VPT>(lds, L, C, d, twy, -1);
@@ -872,14 +873,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         const int cidx = m - c * nay;
         const int gx = gx0 + c;
         if (gx >= hxa) continue;
-        const size_t at = static_cast<size_t>(gx) * nay + cidx;
+        const uint32_t at = static_cast<uint32_t>(gx * nay + cidx);
         if (cidx == half) {   // the always-zero ky = -Nay/2 slot
             if (psi_group) {
-                gstore(P.next[1 + a] + at, czero<T>());
+                gstore(P.next[1 + a], at, czero<T>());
             } else {
-                if (compressible) gstore(P.next[0] + at, czero<T>());
-                gstore(P.next[fx_field] + at, czero<T>());
-                gstore(P.next[fy_field] + at, czero<T>());
+                if (compressible) gstore(P.next[0], at, czero<T>());
+                gstore(P.next[fx_field], at, czero<T>());
+                gstore(P.next[fy_field], at, czero<T>());
             }
             continue;
         }
@@ -896,7 +897,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
 
         if (psi_group) {
             const int f = 1 + a;
-            const cx<T> cur = gload(P.cur[f] + at);
+            const cx<T> cur = gload(P.cur[f], at);
             cx<T> det = acc_x[m];
             if (d.psi_det_on[a]) {
                 if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
@@ -912,7 +913,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
                 for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
                     if (d.igrad_component[q] != a) continue;
-                    const cx<T> j = gload(P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field] + at);
+                    const cx<T> j = gload(P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field], at);
                     const cx<T> t = T(-d.igrad_amplitude[q]) * j;
                     det = det + mk(t.re / T(d.rho0), t.im / T(d.rho0));
                 }
@@ -929,8 +930,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 cx<T> sa, sb = czero<T>();
                 cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
                 if (sp.noise_mode == NOISE_LOAD) {
-                    sa = gload(nz + at);
-                    sb = gload(nz + fe + at);
+                    sa = gload(nz, at);
+                    sb = gload(nz, fe32 + at);
                 } else {
                     const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
                     gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
@@ -938,8 +939,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     sb = sp.use_b ? sigma * sb : czero<T>();
                     if (conj_draw) { sa = conj(sa); sb = conj(sb); }
                     if (sp.noise_mode == NOISE_STORE) {
-                        gstore(nz + at, sa);
-                        gstore(nz + fe + at, sb);
+                        gstore(nz, at, sa);
+                        gstore(nz, fe32 + at, sb);
                     }
                 }
                 sto = sp.use_b ? sa + beta * sb : sa;
@@ -950,14 +951,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
             }
             if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
-            else gstore(P.next[f] + at, wb * gload(P.base[f] + at) + wc * (cur + dt * det + sdt * sto));
+            else gstore(P.next[f], at, wb * gload(P.base[f], at) + wc * (cur + dt * det + sdt * sto));
         } else {
-            const cx<T> jx = gload(P.cur[fx_field] + at);
-            const cx<T> jy = gload(P.cur[fy_field] + at);
+            const cx<T> jx = gload(P.cur[fx_field], at);
+            const cx<T> jy = gload(P.cur[fy_field], at);
             cx<T> dx = acc_x[m], dy = acc_y[m];
             if (d.mom_det_on) {
                 if (d.pressure_coefficient != 0.0) {   // src/fcalculator.cc:276-300
-                    const cx<T> p = T(d.pressure_coefficient) * gload(P.cur[0] + at);
+                    const cx<T> p = T(d.pressure_coefficient) * gload(P.cur[0], at);
                     dx = dx + mul_mi(kx * p);
                     dy = dy + mul_mi(ky * p);
                     if (pi_slot) {
@@ -991,8 +992,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             cx<T> sx = czero<T>(), sy = czero<T>();
             cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
             if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
-                sx = gload(nzj + at) + beta * gload(nzj + 2 * fe + at);
-                sy = gload(nzj + fe + at) + beta * gload(nzj + 3 * fe + at);
+                sx = gload(nzj, at) + beta * gload(nzj, 2 * fe32 + at);
+                sy = gload(nzj, fe32 + at) + beta * gload(nzj, 3 * fe32 + at);
             } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
                 const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
                 const T kxd = conj_draw ? T(0) : kx;
@@ -1017,8 +1018,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                         pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
                     }
                     if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
-                        gstore(nzj + static_cast<size_t>(2 * ab) * fe + at, vx);
-                        gstore(nzj + static_cast<size_t>(2 * ab + 1) * fe + at, vy);
+                        gstore(nzj, (2 * ab) * fe32 + at, vx);
+                        gstore(nzj, (2 * ab + 1) * fe32 + at, vy);
                     }
                     if (ab == 0) { sx = vx; sy = vy; }
                     else { sx = sx + beta * vx; sy = sy + beta * vy; }
@@ -1046,9 +1047,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
                 } else {
-                    gstore(P.next[0] + at, wb * gload(P.base[0] + at) + wc * (gload(P.cur[0] + at) + dt * drho));
-                    gstore(P.next[fx_field] + at, wb * gload(P.base[fx_field] + at) + wc * (jx + dt * dx + sdt * sx));
-                    gstore(P.next[fy_field] + at, wb * gload(P.base[fy_field] + at) + wc * (jy + dt * dy + sdt * sy));
+                    gstore(P.next[0], at, wb * gload(P.base[0], at) + wc * (gload(P.cur[0], at) + dt * drho));
+                    gstore(P.next[fx_field], at, wb * gload(P.base[fx_field], at) + wc * (jx + dt * dx + sdt * sx));
+                    gstore(P.next[fy_field], at, wb * gload(P.base[fy_field], at) + wc * (jy + dt * dy + sdt * sy));
                 }
             } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
                 cx<T> ix = dt * dx + sdt * sx;
@@ -1064,8 +1065,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
                 } else {
-                    gstore(P.next[fx_field] + at, wb * gload(P.base[fx_field] + at) + wc * (jx + ix));
-                    gstore(P.next[fy_field] + at, wb * gload(P.base[fy_field] + at) + wc * (jy + iy));
+                    gstore(P.next[fx_field], at, wb * gload(P.base[fx_field], at) + wc * (jx + ix));
+                    gstore(P.next[fy_field], at, wb * gload(P.base[fy_field], at) + wc * (jy + iy));
                 }
             }
         }
