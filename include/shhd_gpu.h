@@ -287,8 +287,9 @@ typedef struct shhd_tuning {
     int inverse_column_tile;   /* kx columns per inverse-column workgroup     */
     int forward_column_tile;   /* kx columns per forward-column workgroup     */
     int row_lanes;             /* lanes per row-pass workgroup                */
-    int column_transform;      /* column FFTs: 0 fixed-length build where one
-                                  exists, 1 runtime plan radix 8 first,
+    int column_transform;      /* column FFTs: 0 fixed-length build staged
+                                  through LDS where one exists (else the
+                                  runtime plan), 1 runtime plan radix 8 first,
                                   2 runtime plan radix 4, 3 and 4 fused
                                   fixed-length build with one and two LDS
                                   buffers (where one is built for the length
@@ -296,7 +297,10 @@ typedef struct shhd_tuning {
     int row_transform;         /* row FFTs, the same choices                  */
     int plane_block;           /* log2 of the kx columns per block of the
                                   row-to-column planes (0: column-major; a
-                                  block that does not divide Nax/2 uses 0)    */
+                                  block that does not divide Nax/2 uses 0).
+                                  A shape never tuned or set runs the fused
+                                  builds with one LDS buffer (3, 3) and
+                                  blocks of two columns (1).                 */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

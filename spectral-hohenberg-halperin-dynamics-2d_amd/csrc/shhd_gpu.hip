@@ -789,6 +789,18 @@ struct Tuning {
     // that does not divide Nax/2 falls back to 0
     int ulg = 0;
 };
+// launch shapes of a shape that was never tuned: default tiles and lanes,
+// the fused transforms with one LDS buffer, U blocks of two columns (batch
+// instances too: measured faster for the mixed-size ensemble and the served
+// stream than the staged builds, although a batch then launches once per
+// transform length)
+Tuning default_tuning() {
+    Tuning t;
+    t.col_transform = 3;
+    t.row_transform = 3;
+    t.ulg = 1;
+    return t;
+}
 constexpr int kTransforms = 5;
 std::mutex g_tune_mutex;
 std::map<std::string, Tuning> g_tuned;
@@ -913,7 +925,7 @@ shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<In
     const Desc& d = h->desc;
     h->batch_shape = batch_shape;
     h->tune_key = tune_key_of(*h);
-    Tuning t{};
+    Tuning t = default_tuning();
     if (force) t = *force;
     else lookup_tuning(h->tune_key, t);
     apply_plans(*h, t);
@@ -2041,7 +2053,7 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
             if (p1.NT != p0.NT && p1.R > p0.R) row_nts.push_back(p1.NT);
         }
     }
-    Tuning cur{};
+    Tuning cur = default_tuning();
     double ms[kClasses];
     // inverse column tile
     double bk = 1e30;
