@@ -170,7 +170,7 @@ def ensemble_bench(ninst):
     decay = float(np.mean(c[:, 10].real / c[:, 0].real))
     for j in jobs:
         j.probes = None
-    seq_s, _ = serve.run_sequential(jobs)
+    seq_s, _ = serve.run_sequential(jobs, keep_results=False)   # neither side downloads results
     pts = rep.point_steps
     return {"instances": ninst, "steps": 100, "Mpoint_steps_s": pts / rep.makespan / 1e6,
             "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.makespan,
@@ -192,11 +192,15 @@ def served_bench(njobs):
     finally:
         srv.close()
     p50, p99 = rep.latency_percentiles()
-    seq_s, _ = serve.run_sequential(jobs)
+    durations = []
+    seq_s, _ = serve.run_sequential(jobs, keep_results=False, durations=durations)   # neither side downloads
+    fifo = np.array(serve.fifo_latencies(jobs, durations))
     pts = rep.point_steps
     return {"jobs": njobs, "arrival_rate_per_s": 20.0, "p50_s": p50, "p99_s": p99,
             "Mpoint_steps_s": pts / rep.makespan / 1e6, "busy_Mpoint_steps_s": pts / rep.busy / 1e6,
             "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.busy,
+            "sequential_fifo_p50_s": float(np.percentile(fifo, 50)),
+            "sequential_fifo_p99_s": float(np.percentile(fifo, 99)),
             "makespan_s": rep.makespan, "chunks": rep.chunks, "peak_resident": rep.peak_resident}
 
 

@@ -140,19 +140,37 @@ class JobServer:
         return ServeReport(jobs=jobs, makespan=makespan, chunks=chunks, peak_resident=peak, busy=busy)
 
 
-def run_sequential(jobs: List[Job], clock: Callable[[], float] = time.perf_counter):
-    """The same jobs as plain single-simulation runs, one after another."""
+def run_sequential(jobs: List[Job], clock: Callable[[], float] = time.perf_counter, keep_results: bool = True,
+                   durations: Optional[List[float]] = None):
+    """The same jobs as plain single-simulation runs, one after another in
+    arrival order (final states downloaded only with keep_results, as the
+    server does); each job's wall time is appended to `durations` if given."""
     t0 = clock()
     out = []
     for j in sorted(jobs, key=lambda j: (j.arrival, j.job_id)):
+        ts = clock()
         with shhd.GpuFluidSolver(j.cfg) as g:
             if j.make_state:
                 g.set_state(j.make_state())
             for ic in j.ics or []:
                 g.apply_initial_condition(ic)
             g.step(j.nsteps)
-            out.append(g.get_state())
+            out.append(g.get_state() if keep_results else None)
+        if durations is not None:
+            durations.append(clock() - ts)
     return clock() - t0, out
+
+
+def fifo_latencies(jobs: List[Job], durations: List[float]) -> List[float]:
+    """Latencies (finish - arrival) of a one-job-at-a-time FIFO server that
+    receives the jobs at their arrival times and runs each for its measured
+    sequential duration (durations in arrival order, as run_sequential)."""
+    t = 0.0
+    lat = []
+    for j, d in zip(sorted(jobs, key=lambda j: (j.arrival, j.job_id)), durations):
+        t = max(t, j.arrival) + d
+        lat.append(t - j.arrival)
+    return lat
 
 
 # ---------------------------------------------------------------------------
