@@ -98,12 +98,33 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
     }
 }
 
-// Butterfly inputs from LDS, twiddled, transformed in registers.
-template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
-__device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R],
-                                        const cx<T>* __restrict__ tw, int sign, int tid) {
+// Table twiddles of the lane's butterflies for a pass with sub-length NS
+// (issued before the pass's barrier: they do not depend on the LDS data).
+template <typename T, int NT, int L, int NSEQ, int R, int NS>
+struct FzTw {
+    Twiddles<T, R> t[FzMap<NT, L, NSEQ, R>::B];
+};
+template <typename T, int NT, int L, int NSEQ, int R, int NS>
+__device__ __forceinline__ FzTw<T, NT, L, NSEQ, R, NS> fz_twiddles(const cx<T>* __restrict__ tw, int sign, int tid) {
     using Mp = FzMap<NT, L, NSEQ, R>;
     constexpr int TSTEP = L / (NS * R);
+    FzTw<T, NT, L, NSEQ, R, NS> r;
+#pragma unroll
+    for (int b = 0; b < Mp::B; ++b) {
+        const int id = tid + NT * b;
+        const int j = id - (id / Mp::M) * Mp::M;
+        const int k = Mp::valid(id) ? j % NS : 0;
+        r.t[b] = load_twiddles<T, R>(tw, k * TSTEP, sign);
+    }
+    return r;
+}
+
+// Butterfly inputs from LDS, twiddled (twiddles loaded beforehand),
+// transformed in registers.
+template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
+__device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R],
+                                        const FzTw<T, NT, L, NSEQ, R, NS>& twv, int sign, int tid) {
+    using Mp = FzMap<NT, L, NSEQ, R>;
 #pragma unroll
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
@@ -124,11 +145,7 @@ __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
         if (!Mp::valid(id)) continue;
-        if constexpr (NS > 1) {
-            const int j = id - (id / Mp::M) * Mp::M;
-            const int k = j % NS;
-            if (k != 0) apply_twiddles<T, R>(v[b], tw, k * TSTEP, sign);
-        }
+        if constexpr (NS > 1) mul_twiddles<T, R>(v[b], twv.t[b]);
         dft_small<T, R>(v[b], sign);
     }
 }
@@ -158,8 +175,9 @@ __device__ __forceinline__ void fz_pass(FzBufs<T, NBUF>& bf, const cx<T>* __rest
     constexpr int R = fz_radix(L, R3F, P);
     constexpr int NS = fz_ns(L, R3F, P);
     cx<T> v[FzMap<NT, L, NSEQ, R>::B][R];
+    const auto twv = fz_twiddles<T, NT, L, NSEQ, R, NS>(tw, sign, tid);
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P - 1)>(bf.rd(), v, tw, sign, tid);
+    fz_read<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P - 1)>(bf.rd(), v, twv, sign, tid);
     bf.before_write();
     fz_write<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P)>(bf.wr(), v, tid);
     bf.flip();
@@ -189,9 +207,10 @@ __device__ __forceinline__ void fz_rest_to_regs(
     constexpr int NP = fz_npass(L, R3F);
     asm volatile("" : "+v"(tid));
     fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 2>{});
+    constexpr int RL = fz_radix(L, R3F, NP - 1), NSL = fz_ns(L, R3F, NP - 1);
+    const auto twv = fz_twiddles<T, NT, L, NSEQ, RL, NSL>(tw, sign, tid);
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, fz_radix(L, R3F, NP - 1), fz_ns(L, R3F, NP - 1), fz_pad_after(L, R3F, NP - 2)>(
-        bf.rd(), v, tw, sign, tid);
+    fz_read<T, NT, L, NSEQ, RL, NSL, fz_pad_after(L, R3F, NP - 2)>(bf.rd(), v, twv, sign, tid);
 }
 
 }  // namespace shhd

@@ -95,28 +95,46 @@ __device__ __forceinline__ void dft_small(cx<T> (&v)[R], int sign) {
     }
 }
 
-// w^(q t) for q = 1..R-1 from the table tw[j] = exp(-2 pi i j / L): radix 8
-// loads w^t, w^2t, w^4t and forms the other four by products (error growth of
-// at most two roundings), radix 4 loads w^t, w^2t, radix 3 and 2 load all.
+// Twiddles w^(q t), q = 1..R-1, from the table tw[j] = exp(-2 pi i j / L), in
+// two steps so the table loads can be issued before a pass waits at its
+// barrier: load_twiddles fetches w^t, w^2t, w^4t for radix 8 (the other four
+// are formed by products: error growth of at most two roundings), w^t, w^2t for
+// radix 4 and 3, w^t for 2, conjugated for sign > 0; mul_twiddles applies them
+// (t = 0 gives w = 1 exactly).
 template <typename T, int R>
-__device__ __forceinline__ void apply_twiddles(cx<T> (&v)[R], const cx<T>* __restrict__ tw, int t, int sign) {
+struct Twiddles {
+    cx<T> w[R == 8 ? 3 : (R == 2 ? 1 : 2)];
+};
+template <typename T, int R>
+__device__ __forceinline__ Twiddles<T, R> load_twiddles(const cx<T>* __restrict__ tw, int t, int sign) {
     auto ld = [&](int e) {
         cx<T> w = gload(tw + e);
         if (sign > 0) w.im = -w.im;
         return w;
     };
+    Twiddles<T, R> r;
     if constexpr (R == 8) {
-        const cx<T> w1 = ld(t), w2 = ld(2 * t), w4 = ld(4 * t);
+        r.w[0] = ld(t); r.w[1] = ld(2 * t); r.w[2] = ld(4 * t);
+    } else if constexpr (R == 2) {
+        r.w[0] = ld(t);
+    } else {
+        r.w[0] = ld(t); r.w[1] = ld(2 * t);
+    }
+    return r;
+}
+template <typename T, int R>
+__device__ __forceinline__ void mul_twiddles(cx<T> (&v)[R], const Twiddles<T, R>& r) {
+    if constexpr (R == 8) {
+        const cx<T> w1 = r.w[0], w2 = r.w[1], w4 = r.w[2];
         const cx<T> w3 = w1 * w2, w5 = w1 * w4, w6 = w2 * w4, w7 = w3 * w4;
         v[1] = v[1] * w1; v[2] = v[2] * w2; v[3] = v[3] * w3; v[4] = v[4] * w4;
         v[5] = v[5] * w5; v[6] = v[6] * w6; v[7] = v[7] * w7;
     } else if constexpr (R == 4) {
-        const cx<T> w1 = ld(t), w2 = ld(2 * t);
-        v[1] = v[1] * w1; v[2] = v[2] * w2; v[3] = v[3] * (w1 * w2);
+        v[1] = v[1] * r.w[0]; v[2] = v[2] * r.w[1]; v[3] = v[3] * (r.w[0] * r.w[1]);
     } else if constexpr (R == 3) {
-        v[1] = v[1] * ld(t); v[2] = v[2] * ld(2 * t);
+        v[1] = v[1] * r.w[0]; v[2] = v[2] * r.w[1];
     } else {
-        v[1] = v[1] * ld(t);
+        v[1] = v[1] * r.w[0];
     }
 }
 
@@ -143,6 +161,10 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
             for (int q = 0; q < R; ++q) v[b][q] = buf[lp(s * L + j + q * m)];
         }
     }
+    // twiddle loads issued before the barrier (they do not depend on LDS)
+    Twiddles<T, R> twv[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) twv[b] = load_twiddles<T, R>(tw, (jj[b] & ns_mask) * ps.tstep, sign);
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -150,7 +172,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
         if (id < nb) {
             const int j = jj[b];
             const int k = j & ns_mask;
-            if (k != 0) apply_twiddles<T, R>(v[b], tw, k * ps.tstep, sign);
+            if (ps.lns > 0) mul_twiddles<T, R>(v[b], twv[b]);
             dft_small<T, R>(v[b], sign);
             const int out = base[b] + ((j >> ps.lns) << ps.lns) * R + k;
 #pragma unroll
@@ -237,6 +259,12 @@ __device__ __forceinline__ void fft_pass_ct(cx<T>* buf, int nseq, const cx<T>* _
             }
         }
     }
+    // twiddle loads issued before the barrier (they do not depend on LDS)
+    Twiddles<T, R> twv[B];
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) twv[b] = load_twiddles<T, R>(tw, (jj[b] & (NS - 1)) * TSTEP, sign);
+    }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -244,9 +272,7 @@ __device__ __forceinline__ void fft_pass_ct(cx<T>* buf, int nseq, const cx<T>* _
         if (id < nb) {
             const int j = jj[b];
             const int k = j & (NS - 1);
-            if constexpr (NS > 1) {
-                if (k != 0) apply_twiddles<T, R>(v[b], tw, k * TSTEP, sign);
-            }
+            if constexpr (NS > 1) mul_twiddles<T, R>(v[b], twv[b]);
             dft_small<T, R>(v[b], sign);
             const int out = sl[b] + ((j >> LNS) << LNS) * R + k;
             if constexpr (NS % kPad == 0) {
