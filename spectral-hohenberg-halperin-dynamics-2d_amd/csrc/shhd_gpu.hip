@@ -855,6 +855,15 @@ void apply_plans(Instance& h, const Tuning& t) {
     if (fz_nb > 0 && d.my % 3 == 0 && 3 * d.nay <= 2 * d.my) {
         for (ColPlan* p : {&h.cols, &h.cols3}) {
             const int v = 6;
+            // a tile that leaves the lanes half empty (C * My < NT * 6) takes
+            // the columns that fill them when that build exists
+            if (p->C * d.my < p->NT * v && (p->NT * v) % d.my == 0) {
+                const int c2 = p->NT * v / d.my;
+                if ((c2 & (c2 - 1)) == 0 && c2 <= pow2ceil(d.hxa) && col_fused_built(p->NT, v, d.my)) {
+                    p->C = c2;
+                    p->ntiles = (d.hxa + c2 - 1) / c2;
+                }
+            }
             if ((p->C * d.my) % v != 0 || !col_fused_built(p->C * d.my / v, v, d.my)) continue;
             const size_t one = static_cast<size_t>(lp_size(p->C * d.my)) * h.elem;
             p->NT = p->C * d.my / v;

@@ -98,6 +98,26 @@ def test_model_h_n2_srk3():
     check_case(cfg, st, 100)
 
 
+@pytest.mark.parametrize("precision,tol", [(64, FP64_TOL), (32, 1e-5)])
+def test_model_h_512_default_fused_builds(precision, tol):
+    """Model-H N=2 at 512^2 -> 768^2 with the launch shapes of a shape never
+    tuned: fused column builds (768-point columns) and the fused row build
+    with the compressible field layout (768-point rows, two rows per
+    workgroup), against the oracle; 20 SRK3 steps."""
+    banner(f"model-H N=2 512^2 default fused builds fp{precision}")
+    cfg = shhd.SolverConfig(nx_active=512, ny_active=512, dealias="3/2", lx=512.0, ly=512.0, dt=0.005,
+                            num_order_parameters=2, pressure_coefficient=1.0, eta=0.5, zeta=0.5,
+                            mobility=[1.0, 1.0], free_energy_kind=shhd.FREE_ENERGY_PHI4,
+                            mu_k0=[-1.0, -1.0], mu_k2=[1.0, 1.0], phi4_u=[1.0, 1.0],
+                            momentum_advection=True, order_parameter_advection=True, precision=precision)
+    with shhd.GpuFluidSolver(cfg) as g:
+        info = g.info()
+    assert info.inverse_column_fused >= 1 and info.forward_column_fused >= 1 and info.row_fused >= 1
+    rng = np.random.default_rng(5)
+    st = random_active_state(cfg, rng, amp={0: 0.01, 1: 0.3, 2: 0.3, 3: 0.02, 4: 0.02})
+    check_case(cfg, st, 20, tol=tol)
+
+
 def test_model_h_n1_euler_rect():
     """Euler, rectangular grid, Lx != Ly, one order parameter, quadratic + advection."""
     banner("model-H N=1 euler 32x16 3/2 rectangular")
