@@ -299,8 +299,9 @@ typedef struct shhd_tuning {
                                   row-to-column planes (0: column-major; a
                                   block that does not divide Nax/2 uses 0).
                                   A shape never tuned or set runs the fused
-                                  builds with one LDS buffer (3, 3) and
-                                  blocks of two columns (1).                 */
+                                  builds with two LDS buffers where two
+                                  workgroups still fit a CU (4, 4) and blocks
+                                  of four columns (2).                       */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

@@ -43,7 +43,11 @@ std::string g_create_error;
 constexpr int kClasses = 3;
 constexpr size_t kLdsCap = 96 * 1024;
 constexpr size_t kLdsCap3 = 128 * 1024;
-constexpr size_t kLdsMax = 160 * 1024;   // LDS of one CU
+// Two transform buffers only while the workgroups a CU can hold by their waves
+// still fit its 160 KiB of LDS: two of up to 512 lanes, one of 1024 lanes.
+bool two_buffers_fit(size_t one_buffer_bytes, int lanes) {
+    return 2 * one_buffer_bytes <= (lanes >= 1024 ? 160 * 1024 : 80 * 1024);
+}
 constexpr int kMaxSlots = 4096;
 
 struct ColPlan {
@@ -790,15 +794,15 @@ struct Tuning {
     int ulg = 0;
 };
 // launch shapes of a shape that was never tuned: default tiles and lanes,
-// the fused transforms with one LDS buffer, U blocks of two columns (batch
-// instances too: measured faster for the mixed-size ensemble and the served
-// stream than the staged builds, although a batch then launches once per
-// transform length)
+// the fused transforms with two LDS buffers where they fit (else one), U
+// blocks of four columns (batch instances too: measured faster for the
+// mixed-size ensemble and the served stream than the staged builds, although
+// a batch then launches once per transform length)
 Tuning default_tuning() {
     Tuning t;
-    t.col_transform = 3;
-    t.row_transform = 3;
-    t.ulg = 1;
+    t.col_transform = 4;
+    t.row_transform = 4;
+    t.ulg = 2;
     return t;
 }
 constexpr int kTransforms = 5;
@@ -846,7 +850,7 @@ void apply_plans(Instance& h, const Tuning& t) {
     if (rt == 1 || rt == 2) h.rows.fl = 0;
     if (rt >= 3 && h.rows.fl > 0) {
         const size_t one = static_cast<size_t>(lp_size(h.rows.R * d.mx)) * h.elem;
-        h.rows.nbuf = (rt == 4 && 2 * one <= kLdsMax) ? 2 : 1;
+        h.rows.nbuf = (rt == 4 && two_buffers_fit(one, h.rows.NT)) ? 2 : 1;
         h.rows.lds = h.rows.nbuf * one;
     }
     // fused column builds: C * My = NT * V and (K3) every kept ky within the
@@ -870,7 +874,7 @@ void apply_plans(Instance& h, const Tuning& t) {
             p->V = v;
             p->acc = 2 * v / 3;
             p->fz = d.my;
-            p->nbuf = (fz_nb == 2 && 2 * one <= kLdsMax) ? 2 : 1;
+            p->nbuf = (fz_nb == 2 && two_buffers_fit(one, p->NT)) ? 2 : 1;
             p->lds = p->nbuf * one;
             p->lds3 = std::max(p->nbuf * one, static_cast<size_t>(2 * p->C * d.nay) * h.elem);
         }

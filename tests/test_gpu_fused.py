@@ -74,8 +74,9 @@ def test_fused_transforms_match_staged(grid, tiles, rows_fused, fp32):
             out = run(cfg, st, (c, c, 0, ct, rt))
             info = out[2]
             assert info.inverse_column_tile == c and info.forward_column_tile == c
-            assert info.inverse_column_fused == ct - 2 and info.forward_column_fused == ct - 2
-            assert info.row_fused == ((rt - 2) if rows_fused else 0)
+            # two buffers where two workgroups still fit a CU's LDS, else one
+            assert 1 <= info.inverse_column_fused <= ct - 2 and 1 <= info.forward_column_fused <= ct - 2
+            assert (1 <= info.row_fused <= rt - 2) if rows_fused else info.row_fused == 0
             compare(cfg, st, ref, out, tol)
 
 
