@@ -26,10 +26,6 @@
 #include "desc.h"
 #include "fft_fused.h"
 
-#ifndef SHHD_K3_PREFETCH
-#define SHHD_K3_PREFETCH 1
-#endif
-
 namespace shhd {
 
 // Column transforms of K1 and K3: the fixed-length build when the launch
@@ -768,7 +764,6 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         static_assert(FzMap<NT, FZL, CF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
         using M0 = FzMap<NT, FZL, CF, R0>;
         FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
-        constexpr bool PREFETCH = SHHD_K3_PREFETCH != 0;
         // the first butterflies of contribution ci, straight from U
         auto load_contrib = [&](int ci, cx<T> (&v)[M0::B][R0]) {
             const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(g.contrib[ci]) * plane;
@@ -794,9 +789,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
             // the next contribution's loads fly while this one's passes run
             // (the barriers wait for LDS operations only)
-            if constexpr (PREFETCH) {
-                if (ci + 1 < g.ncontrib) load_contrib(ci + 1, v);
-            }
+            if (ci + 1 < g.ncontrib) load_contrib(ci + 1, v);
             cx<T> w[B3][3];
             fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
             // an opaque copy of the lane index: the mode coordinates below are
@@ -813,9 +806,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 if (j < half) accumulate(cf, 2 * b, w[b][0], gx, j, j);
                 if (M3 - j < half) accumulate(cf, 2 * b + 1, w[b][2], gx, j - M3, j - M3 + nay);
             }
-            if constexpr (!PREFETCH) {
-                if (ci + 1 < g.ncontrib) load_contrib(ci + 1, v);
-            }
+
         }
         // park the accumulated terms at their modes once every lane is done
         // with the transform buffers
