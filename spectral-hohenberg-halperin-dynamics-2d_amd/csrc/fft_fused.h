@@ -47,6 +47,9 @@ struct FzMap {
     static constexpr int NB = NSEQ * M;
     static constexpr int B = (NB + NT - 1) / NT;
     __device__ __forceinline__ static bool valid(int id) { return NB % NT == 0 || id < NB; }
+    // sequence and butterfly of a valid id (one sequence: no division)
+    __device__ __forceinline__ static int seq(int id) { return NSEQ == 1 ? 0 : id / M; }
+    __device__ __forceinline__ static int bfly(int id) { return NSEQ == 1 ? id : id - (id / M) * M; }
 };
 
 // Two LDS buffers used alternately (NBUF = 2) or one buffer (NBUF = 1).
@@ -83,8 +86,8 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
         if (!Mp::valid(id)) continue;
-        const int s = id / Mp::M;
-        const int j = id - s * Mp::M;
+        const int s = Mp::seq(id);
+        const int j = Mp::bfly(id);
         const int k = j % NS;
         const int o = s * L + (j - k) * R + k;
         if constexpr (PAD && NS % kPad == 0) {
@@ -112,7 +115,7 @@ __device__ __forceinline__ FzTw<T, NT, L, NSEQ, R, NS> fz_twiddles(const cx<T>* 
 #pragma unroll
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
-        const int j = id - (id / Mp::M) * Mp::M;
+        const int j = Mp::bfly(id);
         const int k = Mp::valid(id) ? j % NS : 0;
         r.t[b] = load_twiddles<T, R>(tw, k * TSTEP, sign);
     }
@@ -129,8 +132,8 @@ __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
         if (!Mp::valid(id)) continue;
-        const int s = id / Mp::M;
-        const int j = id - s * Mp::M;
+        const int s = Mp::seq(id);
+        const int j = Mp::bfly(id);
         const int a = s * L + j;
         if constexpr (PAD && Mp::M % kPad == 0) {
             const int a0 = lp(a);

@@ -97,8 +97,8 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         for (int b = 0; b < M0::B; ++b) {
             const int id = tid + NT * b;
             if (!M0::valid(id)) continue;
-            const int s = id / M0::M;
-            const int j = id - s * M0::M;
+            const int s = M0::seq(id);
+            const int j = M0::bfly(id);
             const int gx = gx0 + s;
 #pragma unroll
             for (int q = 0; q < R0; ++q) {
@@ -289,7 +289,6 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
     if constexpr (NBUF > 0) {
         static_assert(FL > 0 && PPT % 3 == 0, "fused row transforms need a fixed length and radix-3 butterflies per lane");
         constexpr int RF = NT * PPT / FL;              // rows per workgroup
-        constexpr int M3 = FL / 3;
         constexpr int R0 = fz_radix(FL, false, 0);
         using MI = FzMap<NT, FL, RF, R0>;
         constexpr int B3 = PPT / 3;                 // radix-3 butterflies per lane
@@ -311,8 +310,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             for (int b = 0; b < MI::B; ++b) {
                 const int id = lane + NT * b;
                 if (!MI::valid(id)) continue;
-                const int r = id / MI::M;
-                const int j = id - r * MI::M;
+                const int r = MI::seq(id);
+                const int j = MI::bfly(id);
                 const int y = y0 + r;
                 const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
 #pragma unroll
@@ -377,7 +376,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int id = tid + NT * (i / 3);
-                if (F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + id / M3 < my) atomicOr(flag, 1);
+                if (F.make_velocity(i, d.velocity_compressible != 0, r0) &&
+                    y0 + FzMap<NT, FL, RF, 3>::seq(id) < my)
+                    atomicOr(flag, 1);
             }
         }
 
@@ -406,9 +407,11 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             // of one block write one segment per row
             const int nk = RF * hxa;
             const int umask = (1 << d.ulg) - 1;
+            static_assert((RF & (RF - 1)) == 0, "rows per workgroup a power of two");
+            const int lgb = log2p2(RF) + d.ulg;      // log2 of the rows x columns of one U block segment
             for (int idx = tid; idx < nk; idx += NT) {
-                const int kb = idx / (RF << d.ulg);
-                const int rem = idx - kb * (RF << d.ulg);
+                const int kb = idx >> lgb;
+                const int rem = idx - (kb << lgb);
                 const int r = rem >> d.ulg;
                 const int k = (kb << d.ulg) + (rem & umask);
                 const int y = y0 + r;
@@ -773,8 +776,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             for (int b = 0; b < M0::B; ++b) {
                 const int id = lane0 + NT * b;
                 if (!M0::valid(id)) continue;
-                const int s = id / M0::M;
-                const int j = id - s * M0::M;
+                const int s = M0::seq(id);
+                const int j = M0::bfly(id);
                 const int gx = gx0 + s;
                 const uint32_t col = (static_cast<uint32_t>(gx >> d.ulg) * L << d.ulg) + (gx & ((1 << d.ulg) - 1));
 #pragma unroll
@@ -799,8 +802,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
 #pragma unroll
             for (int b = 0; b < B3; ++b) {
                 const int id = lane + NT * b;
-                const int s = id / M3;
-                const int j = id - s * M3;
+                const int s = FzMap<NT, FZL, CF, 3>::seq(id);
+                const int j = FzMap<NT, FZL, CF, 3>::bfly(id);
                 const int gx = gx0 + s;
                 if (gx >= hxa) continue;
                 if (j < half) accumulate(cf, 2 * b, w[b][0], gx, j, j);
@@ -814,8 +817,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
 #pragma unroll
         for (int b = 0; b < B3; ++b) {
             const int id = tid + NT * b;
-            const int s = id / M3;
-            const int j = id - s * M3;
+            const int s = FzMap<NT, FZL, CF, 3>::seq(id);
+            const int j = FzMap<NT, FZL, CF, 3>::bfly(id);
             if (gx0 + s >= hxa) continue;
             if (j < half) {
                 acc_x[s * nay + j] = acc0[2 * b];
