@@ -12,6 +12,8 @@ flux recording and the Hermitian projection on) agree to rounding: 1e-12
 relative L2 in FP64, 1e-5 in the FP32 form.  The staged builds themselves
 are checked against the oracle by the parity tests.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -80,10 +82,11 @@ def test_fused_transforms_match_staged(grid, tiles, rows_fused, fp32):
             compare(cfg, st, ref, out, tol)
 
 
-@pytest.mark.parametrize("mode", [shhd.MODE_INCOMPRESSIBLE, shhd.MODE_QUIESCENT])
-def test_fused_other_modes(mode):
-    print(f"[case] fused transforms mode {mode}", flush=True)
-    cfg = model_h(512, 512, mode=mode)
+@pytest.mark.parametrize("integrator", [shhd.INTEGRATOR_SRK3, shhd.INTEGRATOR_EULER])
+@pytest.mark.parametrize("mode", [shhd.MODE_COMPRESSIBLE, shhd.MODE_INCOMPRESSIBLE, shhd.MODE_QUIESCENT])
+def test_fused_other_modes(mode, integrator):
+    print(f"[case] fused transforms mode {mode} integrator {integrator}", flush=True)
+    cfg = dataclasses.replace(model_h(512, 512, mode=mode), integrator=integrator)
     st = flux_state(cfg)
     ref = run(cfg, st, (2, 2, 0, 0, 0))
     out = run(cfg, st, (2, 2, 0, 4, 4))
