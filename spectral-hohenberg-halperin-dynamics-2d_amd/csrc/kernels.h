@@ -93,21 +93,37 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         const int tid = threadIdx.x;
         FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
         cx<T> v[M0::B][R0];
+        // every load is issued before any is used, at an index inside the
+        // spectra (the last column past the last one, ky = 0 in the padding),
+        // and the padding is zeroed afterwards by selects: a load under a
+        // per-point branch is waited for inside its branch, which serializes
+        // the column's loads
 #pragma unroll
         for (int b = 0; b < M0::B; ++b) {
             const int id = tid + NT * b;
             if (!M0::valid(id)) continue;
-            const int s = M0::seq(id);
+            const int gx = gx0 + M0::seq(id);
             const int j = M0::bfly(id);
-            const int gx = gx0 + s;
+            const uint32_t col = static_cast<uint32_t>((gx < d.hxa ? gx : d.hxa - 1) * d.nay);
 #pragma unroll
             for (int q = 0; q < R0; ++q) {
                 const int ky = signed_gy(j + q * M0::M, FZL);
-                v[b][q] = czero<T>();
-                if (gx < d.hxa && ky < half && -ky < half) {
-                    const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v[b][q] = scale * gload(src, static_cast<uint32_t>(gx * d.nay + cidx));
-                }
+                const bool live = ky < half && -ky < half;
+                const int cidx = live ? (ky >= 0 ? ky : ky + d.nay) : 0;
+                v[b][q] = gload(src, col + static_cast<uint32_t>(cidx));
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < M0::B; ++b) {
+            const int id = tid + NT * b;
+            if (!M0::valid(id)) continue;
+            const bool col_ok = gx0 + M0::seq(id) < d.hxa;
+            const int j = M0::bfly(id);
+#pragma unroll
+            for (int q = 0; q < R0; ++q) {
+                const int ky = signed_gy(j + q * M0::M, FZL);
+                const bool live = col_ok && ky < half && -ky < half;
+                v[b][q] = live ? scale * v[b][q] : czero<T>();
             }
         }
         fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, +1, tid);
@@ -201,27 +217,65 @@ struct RowFields {
             if (alpha == a) v = psi[a][i];
         return v;
     }
-    __device__ __forceinline__ T eval(const Desc& d, OutItem it, int i) const {
+    // Output `it` at every point of the lane: one branch on the output kind
+    // per output, with the points in straight-line code inside each case
+    // (a branch per point splits the arithmetic into blocks of a few
+    // instructions each, which the scheduler cannot interleave).
+    __device__ __forceinline__ void eval_all(const Desc& d, OutItem it, T (&out)[PPT]) const {
         switch (it.kind) {
             case OUT_MU: {
-                const T p = psi_of(it.alpha, i);
-                return T(d.phi4_u[it.alpha]) * p * p * p;   // src/model_free_energy_phi4.cc:54
+                const T u = T(d.phi4_u[it.alpha]);   // src/model_free_energy_phi4.cc:54
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const T p = psi_of(it.alpha, i);
+                    out[i] = u * p * p * p;
+                }
+                break;
             }
-            case OUT_PSIVX: return psi_of(it.alpha, i) * vx(i);
-            case OUT_PSIVY: return psi_of(it.alpha, i) * vy(i);
+            case OUT_PSIVX:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = psi_of(it.alpha, i) * vx(i);
+                break;
+            case OUT_PSIVY:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = psi_of(it.alpha, i) * vy(i);
+                break;
             case OUT_GRADF: {
-                T s = T(0);
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = T(0);
                 for (int g = 0; g < d.n_grad; ++g) {
                     if (d.grad_component[g] != it.alpha) continue;
-                    s += T(-d.grad_amplitude[g]) * (d.grad_direction[g] == 0 ? vx(i) : vy(i));
+                    const T a = T(-d.grad_amplitude[g]);
+                    if (d.grad_direction[g] == 0) {
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) out[i] += a * vx(i);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < PPT; ++i) out[i] += a * vy(i);
+                    }
                 }
-                return s;
+                break;
             }
-            case OUT_VX: return vx(i);
-            case OUT_VY: return vy(i);
-            case OUT_PXX: return jx[i] * vx(i);
-            case OUT_PXY: return jx[i] * vy(i);
-            default: return jy[i] * vy(i);
+            case OUT_VX:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = vx(i);
+                break;
+            case OUT_VY:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = vy(i);
+                break;
+            case OUT_PXX:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = jx[i] * vx(i);
+                break;
+            case OUT_PXY:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = jx[i] * vy(i);
+                break;
+            default:
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) out[i] = jy[i] * vy(i);
+                break;
         }
     }
 };
@@ -302,17 +356,22 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
             const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
             cx<T> v[MI::B][R0];
+            cx<T> vb[MI::B][R0];
             // an opaque copy of the lane index: the load offsets are formed
             // per transform, not held live across the loop
             int lane = tid;
             asm volatile("" : "+v"(lane));
+            // every load is issued before any is used, at an index that is
+            // always inside the row (kx = 0 for the zero-padded points and
+            // row 0 past the last row), and the padding is zeroed afterwards
+            // by selects: a load under a per-point branch is waited for
+            // inside its branch, which serializes the row's loads
 #pragma unroll
             for (int b = 0; b < MI::B; ++b) {
                 const int id = lane + NT * b;
                 if (!MI::valid(id)) continue;
-                const int r = MI::seq(id);
+                const int y = y0 + MI::seq(id);
                 const int j = MI::bfly(id);
-                const int y = y0 + r;
                 const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
 #pragma unroll
                 for (int q = 0; q < R0; ++q) {
@@ -320,16 +379,29 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     // kx = FL - k (k > FL - Nax/2) of the Hermitian row
                     const int k = j + q * MI::M;
                     const bool mirror = k > FL - hxa;
-                    const int kk = mirror ? FL - k : k;
-                    cx<T> z = czero<T>();
-                    if (y < my && (k < hxa || mirror)) {
-                        cx<T> a = gload(ta, row + kk);
-                        cx<T> bb = has_b ? gload(tb, row + kk) : czero<T>();
-                        if (k == 0) { a.im = T(0); bb.im = T(0); }
-                        if (mirror) { a.im = -a.im; bb.im = -bb.im; }   // conj(A) + i conj(B)
-                        z = mk(a.re - bb.im, a.im + bb.re);             // A + iB
-                    }
-                    v[b][q] = z;
+                    const int kk = mirror ? FL - k : (k < hxa ? k : 0);
+                    v[b][q] = gload(ta, row + kk);
+                    vb[b][q] = gload(tb, row + kk);   // tb = ta when there is no second field
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < MI::B; ++b) {
+                const int id = lane + NT * b;
+                if (!MI::valid(id)) continue;
+                const bool row_ok = y0 + MI::seq(id) < my;
+                const int j = MI::bfly(id);
+#pragma unroll
+                for (int q = 0; q < R0; ++q) {
+                    const int k = j + q * MI::M;
+                    const bool mirror = k > FL - hxa;
+                    const bool live = row_ok && (k < hxa || mirror);
+                    cx<T> a = v[b][q];
+                    cx<T> bb = has_b ? vb[b][q] : czero<T>();
+                    // the kx = 0 mode is real; the mirror half is conj(A) + i conj(B)
+                    a.im = k == 0 ? T(0) : (mirror ? -a.im : a.im);
+                    bb.im = k == 0 ? T(0) : (mirror ? -bb.im : bb.im);
+                    const cx<T> z = mk(a.re - bb.im, a.im + bb.re);   // A + iB
+                    v[b][q] = live ? z : czero<T>();
                 }
             }
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
@@ -388,15 +460,18 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const OutItem ia = d.out[o];
             const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
             F.fence();
+            T va[PPT], vb[PPT];
+            F.eval_all(d, ia, va);
+            if (has_b) F.eval_all(d, ib, vb);
+            else {
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) vb[i] = T(0);
+            }
             cx<T> v[B3][3];
 #pragma unroll
             for (int b = 0; b < B3; ++b)
 #pragma unroll
-                for (int p = 0; p < 3; ++p) {
-                    const T va = F.eval(d, ia, 3 * b + p);
-                    const T vb = has_b ? F.eval(d, ib, 3 * b + p) : T(0);
-                    v[b][p] = mk(va, vb);
-                }
+                for (int p = 0; p < 3; ++p) v[b][p] = mk(va[3 * b + p], vb[3 * b + p]);
             fz_first<T, NT, FL, RF, true, NBUF>(bf, v, -1, tid);
             fz_rest_to_lds<T, NT, FL, RF, true, NBUF>(bf, twx, -1, tid);
             __syncthreads();
@@ -486,14 +561,17 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         const OutItem ia = d.out[o];
         const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
         F.fence();
+        T va[PPT], vb[PPT];
+        F.eval_all(d, ia, va);
+        if (has_b) F.eval_all(d, ib, vb);
+        else {
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) vb[i] = T(0);
+        }
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const int idx = tid + NT * i;
-            if (idx < npts) {
-                const T va = F.eval(d, ia, i);
-                const T vb = has_b ? F.eval(d, ib, i) : T(0);
-                lds[lp(idx)] = mk(va, vb);
-            }
+            if (idx < npts) lds[lp(idx)] = mk(va[i], vb[i]);
         }
         __syncthreads();
         row_fft<T, NT, PPT, FL>(lds, L, R, d.plan_x, twx, -1);
