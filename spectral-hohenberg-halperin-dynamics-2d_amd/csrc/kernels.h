@@ -663,10 +663,13 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
 // ---------------------------------------------------------------------------
 // noise: two complex N(0,1) deviates per Philox call.  The four 32-bit words
 // give two Box-Muller pairs, each from a radius uniform u1 = (r + 1/2) 2^-32
-// (so the tails reach sqrt(-2 ln 2^-33) = 6.8 sigma) and an angle uniform;
-// the transcendentals run in single precision in both arithmetic types
-// (the deviates' law is the normal law to ~1e-7, far below any sampling
-// error a run can resolve).
+// (so the tails reach sqrt(-2 ln 2^-33) = 6.8 sigma) and an angle uniform
+// u2 = r' 2^-32.  The transcendentals are the hardware single-precision
+// instructions in both arithmetic types: base-2 logarithm (u1 lies in
+// [2^-33, 1], never denormal), square root, and sine and cosine, which take
+// their argument in revolutions, so sin(2 pi u2) is one instruction on u2.
+// The deviates agree with a double-precision evaluation of the same formula
+// to ~1e-6, far below any sampling error a run can resolve.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void gauss_quad(const Desc& d, uint32_t mode_id, uint32_t stream, const StageParams& sp,
@@ -678,11 +681,12 @@ __device__ __forceinline__ void gauss_quad(const Desc& d, uint32_t mode_id, uint
     ctr.v[3] = sp.step_hi;
     const u32x4 r = philox4x32_10(ctr, d.seed_lo, d.seed_hi);
     auto box_muller = [](uint32_t a, uint32_t b) {
+        constexpr float kMinus2Ln2 = -1.38629436111989061883f;   // -2 ln 2: -2 ln u = -2 ln2 log2 u
         const float u1 = (static_cast<float>(a) + 0.5f) * 0x1.0p-32f;
         const float u2 = static_cast<float>(b) * 0x1.0p-32f;
-        const float rad = sqrtf(-2.0f * logf(u1));
-        float sn, cs;
-        sincospif(2.0f * u2, &sn, &cs);
+        const float rad = __builtin_amdgcn_sqrtf(kMinus2Ln2 * __builtin_amdgcn_logf(u1));
+        const float sn = __builtin_amdgcn_sinf(u2);
+        const float cs = __builtin_amdgcn_cosf(u2);
         return mk(static_cast<T>(rad * cs), static_cast<T>(rad * sn));
     };
     z0 = box_muller(r.v[0], r.v[1]);

@@ -10,6 +10,7 @@ import pytest
 
 import shhd
 import oracle_py as op
+import noise_ref as nr
 
 pytestmark = pytest.mark.gpu
 
@@ -167,3 +168,79 @@ def test_oracle_and_gpu_same_noise_law():
     r = vg[mask].sum() / vo[mask].sum()
     print(f"    gpu/oracle variance ratio {r:.4f}", flush=True)
     assert abs(r - 1) < 0.03
+
+
+def noise_step(cfg, rho_one):
+    """The state after one Euler step from zero (rho = 1 when rho_one): with
+    every deterministic term zero, the increment is the step's noise alone."""
+    zero = np.zeros(cfg.state_shape(), complex)
+    if rho_one:
+        zero[0, 0, 0] = cfg.nx * cfg.ny
+    with shhd.GpuFluidSolver(cfg) as g:
+        g.set_state(zero)
+        g.step(1)
+        return g.get_state()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_psi_deviates_are_philox_box_muller(precision):
+    """Mode by mode, the psi noise of the first step is sqrt(dt) sigma_k times
+    the first Box-Muller deviate of Philox4x32-10 at counter (mode, 0, 0, 0):
+    the generator, its counter layout, the seed split and the kx = 0
+    conjugate pairing, against the CPU restatement pinned by its
+    known-answer vectors."""
+    banner(f"psi deviates fp{precision}")
+    seed = 0x1_2345_6789
+    cfg = shhd.SolverConfig(nx_active=64, ny_active=64, dealias="3/2", lx=40.0, ly=40.0, dt=0.01,
+                            integrator=shhd.INTEGRATOR_EULER, mode=shhd.MODE_QUIESCENT, num_order_parameters=1,
+                            mobility=[0.6], noise_order_parameter=True, kBT=0.3, order_parameter_noise_chi=2.0,
+                            seed=seed, precision=precision)
+    got = noise_step(cfg, False)[1]
+    kx, ky, _ = shhd.wavenumbers(cfg)
+    nc = cfg.nx * cfg.ny
+    scale = math.sqrt(cfg.dt) * np.sqrt(0.5 * (2 * 0.3 * 2.0 * nc * nc / 1600.0) * 0.6 * (kx ** 2 + ky ** 2))
+    mode, conj, live = nr.mode_table(cfg)
+    z0, _ = nr.gauss_quad(mode, 0, 0, seed)
+    want = scale * np.where(conj, np.conj(z0), z0)
+    err = np.abs(got - want)[live] / scale[live]
+    print(f"    {live.sum()} modes, max deviate error {err.max():.2e}", flush=True)
+    assert err.max() < 1e-4
+    assert np.all(got[~live] == 0)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_momentum_deviates_are_philox_box_muller(precision):
+    """The momentum noise of the first Euler step from rest, mode by mode:
+    the stress deviates x1, x2 (both pairs of stream 8) and x3 (first pair of
+    stream 10) combined into -i k . Pi with Pi_xx, Pi_yy = -s (sqrt(zeta) x1
+    +- sqrt(eta) x2), Pi_xy = -s sqrt(eta) x3 (src/fcalculator.cc:586-700)."""
+    banner(f"momentum deviates fp{precision}")
+    seed = 77
+    cfg = shhd.SolverConfig(nx_active=64, ny_active=64, dealias="3/2", lx=40.0, ly=40.0, dt=0.01,
+                            integrator=shhd.INTEGRATOR_EULER, eta=0.3, zeta=0.7, noise_momentum=True, kBT=0.5,
+                            seed=seed, precision=precision)
+    s = noise_step(cfg, True)
+    kx, ky, _ = shhd.wavenumbers(cfg)
+    kx = np.broadcast_to(kx, s[1].shape)
+    ky = np.broadcast_to(ky, s[1].shape)
+    nc = cfg.nx * cfg.ny
+    ss = math.sqrt(0.5 * 2 * 0.5 * nc * nc / 1600.0)
+    se, sz = math.sqrt(0.3), math.sqrt(0.7)
+    mode, conj, live = nr.mode_table(cfg)
+    x1, x2 = nr.gauss_quad(mode, 8, 0, seed)
+    x3, _ = nr.gauss_quad(mode, 10, 0, seed)
+    kxd = np.where(conj, 0.0, kx)
+    kyd = np.where(conj, -ky, ky)
+    pxx = -ss * (sz * x1 + se * x2)
+    pyy = -ss * (sz * x1 - se * x2)
+    pxy = -ss * se * x3
+    vx = -1j * (kxd * pxx + kyd * pxy)
+    vy = -1j * (kxd * pxy + kyd * pyy)
+    vx = np.where(conj, np.conj(vx), vx)
+    vy = np.where(conj, np.conj(vy), vy)
+    scale = math.sqrt(cfg.dt) * ss * np.sqrt(kx ** 2 + ky ** 2)
+    for f, v in ((1, vx), (2, vy)):
+        err = np.abs(s[f] - math.sqrt(cfg.dt) * v)[live] / scale[live]
+        print(f"    field {f}: max deviate error {err.max():.2e}", flush=True)
+        assert err.max() < 1e-4
+        assert np.all(s[f][~live] == 0)
