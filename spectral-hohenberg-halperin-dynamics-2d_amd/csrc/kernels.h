@@ -988,9 +988,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         const bool conj_draw = (gx == 0 && ky_i < 0);
         const uint32_t mode_id = static_cast<uint32_t>(gx * nay + (conj_draw ? -ky_i : cidx));
 
+        // every load of the mode is issued here, ahead of the noise draws and
+        // the flux updates (whose stores the later loads could not pass);
+        // the base state is read only when its weight is nonzero
+        const bool use_base = wb != T(0);
         if (psi_group) {
             const int f = 1 + a;
             const cx<T> cur = gload(P.cur[f], at);
+            const cx<T> bas = use_base ? gload(P.base[f], at) : czero<T>();
             cx<T> det = acc_x[m];
             if (d.psi_det_on[a]) {
                 if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
@@ -1044,14 +1049,19 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 }
             }
             if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
-            else gstore(P.next[f], at, wb * gload(P.base[f], at) + wc * (cur + dt * det + sdt * sto));
+            else gstore(P.next[f], at, wb * bas + wc * (cur + dt * det + sdt * sto));
         } else {
+            const bool pressure = d.mom_det_on && d.pressure_coefficient != 0.0;
             const cx<T> jx = gload(P.cur[fx_field], at);
             const cx<T> jy = gload(P.cur[fy_field], at);
+            const cx<T> rho = (compressible || pressure) ? gload(P.cur[0], at) : czero<T>();
+            const cx<T> bx = use_base ? gload(P.base[fx_field], at) : czero<T>();
+            const cx<T> by = use_base ? gload(P.base[fy_field], at) : czero<T>();
+            const cx<T> brho = (use_base && compressible) ? gload(P.base[0], at) : czero<T>();
             cx<T> dx = acc_x[m], dy = acc_y[m];
             if (d.mom_det_on) {
-                if (d.pressure_coefficient != 0.0) {   // src/fcalculator.cc:276-300
-                    const cx<T> p = T(d.pressure_coefficient) * gload(P.cur[0], at);
+                if (pressure) {   // src/fcalculator.cc:276-300
+                    const cx<T> p = T(d.pressure_coefficient) * rho;
                     dx = dx + mul_mi(kx * p);
                     dy = dy + mul_mi(ky * p);
                     if (pi_slot) {
@@ -1140,9 +1150,9 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
                 } else {
-                    gstore(P.next[0], at, wb * gload(P.base[0], at) + wc * (gload(P.cur[0], at) + dt * drho));
-                    gstore(P.next[fx_field], at, wb * gload(P.base[fx_field], at) + wc * (jx + dt * dx + sdt * sx));
-                    gstore(P.next[fy_field], at, wb * gload(P.base[fy_field], at) + wc * (jy + dt * dy + sdt * sy));
+                    gstore(P.next[0], at, wb * brho + wc * (rho + dt * drho));
+                    gstore(P.next[fx_field], at, wb * bx + wc * (jx + dt * dx + sdt * sx));
+                    gstore(P.next[fy_field], at, wb * by + wc * (jy + dt * dy + sdt * sy));
                 }
             } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
                 cx<T> ix = dt * dx + sdt * sx;
@@ -1158,8 +1168,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
                     k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
                 } else {
-                    gstore(P.next[fx_field], at, wb * gload(P.base[fx_field], at) + wc * (jx + ix));
-                    gstore(P.next[fy_field], at, wb * gload(P.base[fy_field], at) + wc * (jy + iy));
+                    gstore(P.next[fx_field], at, wb * bx + wc * (jx + ix));
+                    gstore(P.next[fy_field], at, wb * by + wc * (jy + iy));
                 }
             }
         }

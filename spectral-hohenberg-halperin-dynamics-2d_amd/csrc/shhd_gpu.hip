@@ -2132,16 +2132,19 @@ double kernel_bytes_of(const Instance& h, int cls) {
     const double pb = static_cast<double>(h.plane_elems) * static_cast<double>(h.elem);
     if (cls == 0) return d.n_in * (fb + pb);
     if (cls == 1) return (d.n_in + d.n_out) * pb;
+    // the base state is read only by the stages that weight it (SRK3 stages
+    // 2 and 3; never in Euler): its share of a launch, averaged over a step
+    const double base = h.stages == 3 ? 2.0 / 3.0 : 0.0;
     double k3 = d.n_out * pb;
     for (int g = 0; g < d.ngroups; ++g) {
         const Group& G = d.groups[g];
         if (G.kind == GROUP_PSI) {
-            k3 += 3.0 * fb;   // cur, base, next of psi_a
+            k3 += (2.0 + base) * fb;   // cur, next (and base) of psi_a
             k3 += static_cast<double>(d.n_igrad > 0 ? 2 : 0) * fb;
         } else if (d.mode == SHHD_MODE_COMPRESSIBLE) {
-            k3 += 9.0 * fb;   // rho, jx, jy: cur, base, next
+            k3 += 3.0 * (2.0 + base) * fb;   // rho, jx, jy: cur, next (and base)
         } else {
-            k3 += 6.0 * fb + (d.pressure_coefficient != 0.0 ? fb : 0.0);
+            k3 += 2.0 * (2.0 + base) * fb + (d.pressure_coefficient != 0.0 ? fb : 0.0);
         }
     }
     return k3;
