@@ -165,7 +165,8 @@ enum FluxBits : int { FLUX_DENSITY = 1, FLUX_ORDER_PARAMETER = 2, FLUX_MOMENTUM 
 template <typename T>
 struct InstDev {
     Desc d;
-    const cx<T>* twx;         // twiddles exp(-2 pi i j / Mx), j < Mx
+    const cx<T>* twx;         // twiddles exp(-2 pi i j / Mx), j < Mx, then the
+                              // radix-8 rows of the fused transforms (fz_tw8_at)
     const cx<T>* twy;         // same for My
     cx<T>* tbuf;              // K1 -> K2 planes, row-major: [field][y][kx]
     cx<T>* ubuf;              // K2 -> K3 planes, kx-blocked column-major (Desc::ulg)

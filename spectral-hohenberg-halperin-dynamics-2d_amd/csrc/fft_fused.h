@@ -19,6 +19,8 @@
 // the forward transform through registers.
 #pragma once
 
+#include <type_traits>
+
 #include "lds_fft.h"
 
 namespace shhd {
@@ -37,6 +39,27 @@ constexpr int fz_ns(int L, bool r3_first, int P) {
     for (int p = 0; p < P; ++p) ns *= fz_radix(L, r3_first, p);
     return ns;
 }
+
+// Twiddles of the radix-8 passes, appended to the flat table
+// tw[j] = exp(-2 pi i j / L), j < L: for the radix-3-last order, then the
+// radix-3-first order, every radix-8 pass P >= 1 of sub-length ns holds 8 ns
+// values, the seven twiddles w^(q k L / (8 ns)), q = 1..7, of each butterfly
+// index k < ns (copies of the flat table's entries).  A butterfly loads all
+// seven instead of forming four of them as products of three.  The values of
+// one q lie in a plane over k, so the lanes of a wave (consecutive k) load
+// consecutive entries: in FP64 plane q - 1 at (q - 1) ns + k; in FP32 planes
+// of pairs, (w^q, w^(q+1)) for odd q at 2 (((q - 1) / 2) ns + k), one 16-byte
+// load (the pair of q = 7 is padded).
+constexpr int fz_tw8_count(int L, bool r3_first, int P) {   // butterfly indices of the radix-8 passes before P
+    int n = 0;
+    for (int p = 1; p < P; ++p)
+        if (fz_radix(L, r3_first, p) == 8) n += fz_ns(L, r3_first, p);
+    return n;
+}
+constexpr int fz_tw8_at(int L, bool r3_first, int P) {
+    return L + 8 * ((r3_first ? fz_tw8_count(L, false, fz_npass(L, false)) : 0) + fz_tw8_count(L, r3_first, P));
+}
+constexpr int fz_tw_size(int L) { return fz_tw8_at(L, true, fz_npass(L, true)); }
 
 // Butterflies of a radix-R pass over NSEQ sequences of L points on NT lanes:
 // lane t holds butterflies id = t + NT b, b < B; id = s M + j (sequence s,
@@ -101,14 +124,57 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
     }
 }
 
-// Table twiddles of the lane's butterflies for a pass with sub-length NS
-// (issued before the pass's barrier: they do not depend on the LDS data).
+// The seven twiddles of butterfly index k of a radix-8 pass whose planes
+// start at `pass` (sub-length NS).
+template <typename T>
+struct Tw8 {
+    cx<T> w[7];
+};
+template <typename T, int NS>
+__device__ __forceinline__ Tw8<T> load_tw8(const cx<T>* __restrict__ pass, int k, int sign) {
+    Tw8<T> r;
+    if constexpr (sizeof(T) == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) f4* p = (const __attribute__((address_space(1))) f4*)(pass);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const f4 v = p[h * NS + k];
+            r.w[2 * h] = mk(v.x, v.y);
+            if (h < 3) r.w[2 * h + 1] = mk(v.z, v.w);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) r.w[q] = gload(pass + q * NS + k);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (sign > 0) r.w[q].im = -r.w[q].im;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ void mul_tw8(cx<T> (&v)[8], const Tw8<T>& r) {
+#pragma unroll
+    for (int q = 1; q < 8; ++q) v[q] = v[q] * r.w[q - 1];
+}
+
+// Table twiddles of the lane's butterflies for pass P (sub-length NS) of the
+// given radix order, issued before the pass's barrier (they do not depend on
+// the LDS data): the appended planes for radix 8, the flat table
+// (w^t, w^2t, and w^4t with products for the rest) otherwise.
+// (The FP64 builds of 1024 lanes keep the three-load form: their registers
+// are already short, and seven twiddles held across the barrier cost more
+// than the four products save.)
+template <typename T, int NT, int R>
+constexpr bool fz_tw8_planes() { return R == 8 && (sizeof(T) == 4 || NT < 1024); }
 template <typename T, int NT, int L, int NSEQ, int R, int NS>
 struct FzTw {
-    Twiddles<T, R> t[FzMap<NT, L, NSEQ, R>::B];
+    using W = std::conditional_t<fz_tw8_planes<T, NT, R>(), Tw8<T>, Twiddles<T, R>>;
+    W t[FzMap<NT, L, NSEQ, R>::B];
 };
-template <typename T, int NT, int L, int NSEQ, int R, int NS>
-__device__ __forceinline__ FzTw<T, NT, L, NSEQ, R, NS> fz_twiddles(const cx<T>* __restrict__ tw, int sign, int tid) {
+template <typename T, int NT, int L, int NSEQ, bool R3F, int P>
+__device__ __forceinline__ auto fz_twiddles(const cx<T>* __restrict__ tw, int sign, int tid) {
+    constexpr int R = fz_radix(L, R3F, P);
+    constexpr int NS = fz_ns(L, R3F, P);
     using Mp = FzMap<NT, L, NSEQ, R>;
     constexpr int TSTEP = L / (NS * R);
     FzTw<T, NT, L, NSEQ, R, NS> r;
@@ -117,7 +183,8 @@ __device__ __forceinline__ FzTw<T, NT, L, NSEQ, R, NS> fz_twiddles(const cx<T>* 
         const int id = tid + NT * b;
         const int j = Mp::bfly(id);
         const int k = Mp::valid(id) ? j % NS : 0;
-        r.t[b] = load_twiddles<T, R>(tw, k * TSTEP, sign);
+        if constexpr (fz_tw8_planes<T, NT, R>()) r.t[b] = load_tw8<T, NS>(tw + fz_tw8_at(L, R3F, P), k, sign);
+        else r.t[b] = load_twiddles<T, R>(tw, k * TSTEP, sign);
     }
     return r;
 }
@@ -148,7 +215,8 @@ __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
         if (!Mp::valid(id)) continue;
-        if constexpr (NS > 1) mul_twiddles<T, R>(v[b], twv.t[b]);
+        if constexpr (NS > 1 && fz_tw8_planes<T, NT, R>()) mul_tw8<T>(v[b], twv.t[b]);
+        else if constexpr (NS > 1) mul_twiddles<T, R>(v[b], twv.t[b]);
         dft_small<T, R>(v[b], sign);
     }
 }
@@ -178,7 +246,7 @@ __device__ __forceinline__ void fz_pass(FzBufs<T, NBUF>& bf, const cx<T>* __rest
     constexpr int R = fz_radix(L, R3F, P);
     constexpr int NS = fz_ns(L, R3F, P);
     cx<T> v[FzMap<NT, L, NSEQ, R>::B][R];
-    const auto twv = fz_twiddles<T, NT, L, NSEQ, R, NS>(tw, sign, tid);
+    const auto twv = fz_twiddles<T, NT, L, NSEQ, R3F, P>(tw, sign, tid);
     __syncthreads();
     fz_read<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P - 1)>(bf.rd(), v, twv, sign, tid);
     bf.before_write();
@@ -211,7 +279,7 @@ __device__ __forceinline__ void fz_rest_to_regs(
     asm volatile("" : "+v"(tid));
     fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 2>{});
     constexpr int RL = fz_radix(L, R3F, NP - 1), NSL = fz_ns(L, R3F, NP - 1);
-    const auto twv = fz_twiddles<T, NT, L, NSEQ, RL, NSL>(tw, sign, tid);
+    const auto twv = fz_twiddles<T, NT, L, NSEQ, R3F, NP - 1>(tw, sign, tid);
     __syncthreads();
     fz_read<T, NT, L, NSEQ, RL, NSL, fz_pad_after(L, R3F, NP - 2)>(bf.rd(), v, twv, sign, tid);
 }

@@ -318,11 +318,26 @@ shhd_status set_err(std::nullptr_t, shhd_status st, const std::string& msg) {
 
 template <typename T>
 void make_twiddles(int L, std::vector<cx<T>>& out) {
-    out.resize(static_cast<size_t>(L));
+    out.resize(static_cast<size_t>(fz_tw_size(L)));
     const double pi = 3.14159265358979323846;
     for (int j = 0; j < L; ++j) {
         const double a = 2.0 * pi * static_cast<double>(j) / static_cast<double>(L);
         out[static_cast<size_t>(j)] = mk(static_cast<T>(std::cos(a)), static_cast<T>(-std::sin(a)));
+    }
+    // the radix-8 twiddle planes of the fused transforms (layout in fft_fused.h)
+    for (const bool r3f : {false, true}) {
+        for (int P = 1; P < fz_npass(L, r3f); ++P) {
+            if (fz_radix(L, r3f, P) != 8) continue;
+            const int ns = fz_ns(L, r3f, P);
+            const int tstep = L / (8 * ns);
+            const size_t at = static_cast<size_t>(fz_tw8_at(L, r3f, P));
+            for (int k = 0; k < ns; ++k)
+                for (int q = 1; q < 8; ++q) {
+                    const size_t e = sizeof(T) == 4 ? 2 * static_cast<size_t>(((q - 1) / 2) * ns + k) + (q - 1) % 2
+                                                    : static_cast<size_t>((q - 1) * ns + k);
+                    out[at + e] = out[static_cast<size_t>(q * k * tstep)];
+                }
+        }
     }
 }
 
