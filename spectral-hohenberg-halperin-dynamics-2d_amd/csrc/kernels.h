@@ -133,17 +133,26 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
         __syncthreads();
         res = bf.rd();
     } else {
-        for (int c = 0; c < C; ++c) {
+        // element e = c L + y of the tile (C L <= NT VPT, host-checked); all
+        // loads issued first at indices inside the spectra, masks applied after
+        cx<T> v[VPT];
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int e = static_cast<int>(threadIdx.x) + NT * i;
+            const int c = e / L;
+            const int ky = signed_gy(e - c * L, L);
             const int gx = gx0 + c;
-            for (int y = threadIdx.x; y < L; y += NT) {
-                cx<T> v = czero<T>();
-                const int ky = signed_gy(y, L);
-                if (gx < d.hxa && ky < half && -ky < half) {
-                    const int cidx = ky >= 0 ? ky : ky + d.nay;
-                    v = scale * gload(src, static_cast<uint32_t>(gx * d.nay + cidx));
-                }
-                lds[lp(c * L + y)] = v;
-            }
+            const int cidx = (ky < half && -ky < half) ? (ky >= 0 ? ky : ky + d.nay) : 0;
+            v[i] = gload(src, static_cast<uint32_t>((gx < d.hxa ? gx : d.hxa - 1) * d.nay + cidx));
+        }
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int e = static_cast<int>(threadIdx.x) + NT * i;
+            if (e >= ncol) continue;
+            const int c = e / L;
+            const int ky = signed_gy(e - c * L, L);
+            const bool live = gx0 + c < d.hxa && ky < half && -ky < half;
+            lds[lp(e)] = live ? scale * v[i] : czero<T>();
         }
         __syncthreads();
         col_fft<T, NT, VPT>(lds, L, C, d, twy, +1);
@@ -509,26 +518,36 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         const bool has_b = s + 1 < d.n_in;
         const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
         const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
-        for (int idx = tid; idx < npts; idx += NT) {
+        // point idx = r L + k of the rows (R L <= NT PPT, host-checked); all
+        // loads issued first at indices inside the rows (kx = 0 for the
+        // zero-padded points, row 0 past the last row), masks applied after
+        cx<T> va[PPT], vb[PPT];
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int idx = tid + NT * i;
             const int r = idx / L;
             const int k = idx - r * L;
             const int y = y0 + r;
-            cx<T> z = czero<T>();
-            if (y < my) {
-                const uint32_t row = static_cast<uint32_t>(y * hxa);
-                if (k < hxa) {
-                    cx<T> a = gload(ta, row + k);
-                    cx<T> b = has_b ? gload(tb, row + k) : czero<T>();
-                    if (k == 0) { a.im = T(0); b.im = T(0); }
-                    z = mk(a.re - b.im, a.im + b.re);               // A + iB
-                } else if (k > L - hxa) {
-                    const int kk = L - k;
-                    const cx<T> a = gload(ta, row + kk);
-                    const cx<T> b = has_b ? gload(tb, row + kk) : czero<T>();
-                    z = mk(a.re + b.im, b.re - a.im);               // conj(A) + i conj(B)
-                }
-            }
-            lds[lp(idx)] = z;
+            const bool mirror = k > L - hxa;
+            const int kk = mirror ? L - k : (k < hxa ? k : 0);
+            const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
+            va[i] = gload(ta, row + kk);
+            vb[i] = gload(tb, row + kk);   // tb = ta when there is no second field
+        }
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int idx = tid + NT * i;
+            if (idx >= npts) continue;
+            const int r = idx / L;
+            const int k = idx - r * L;
+            const bool mirror = k > L - hxa;
+            const bool live = y0 + r < my && (k < hxa || mirror);
+            cx<T> a = va[i];
+            cx<T> b = has_b ? vb[i] : czero<T>();
+            // the kx = 0 mode is real; the mirror half is conj(A) + i conj(B)
+            a.im = k == 0 ? T(0) : (mirror ? -a.im : a.im);
+            b.im = k == 0 ? T(0) : (mirror ? -b.im : b.im);
+            lds[lp(idx)] = live ? mk(a.re - b.im, a.im + b.re) : czero<T>();   // A + iB
         }
         __syncthreads();
         row_fft<T, NT, PPT, FL>(lds, L, R, d.plan_x, twx, +1);
