@@ -153,10 +153,11 @@ def time_gpu(cfg, steps, warmup, ics=None):
 
 def ensemble_bench(ninst):
     """Batched ensemble (the quiescent-ensemble workload): ninst quiescent phi4 realizations
-    with sizes drawn from {64,128,256,512}^2, 100 SRK3 steps each (the stated
-    10^4 scaled down), dx = 1.5 for stability; all advanced together (one
-    launch per kernel, stage and launch shape, hipGraph replay) against the
-    same runs done one after another with single handles."""
+    with sizes drawn from {64,128,256,512}^2, each from psi = 0 set on the
+    device, 100 SRK3 steps each (the stated 10^4 scaled down), dx = 1.5 for
+    stability; all advanced together (one launch per kernel, stage and launch
+    shape, hipGraph replay) against the same runs done one after another with
+    single handles."""
     import serve
     import measures
     jobs = serve.ensemble(ninst, nsteps=100, low_modes=16)

@@ -260,11 +260,22 @@ def job_ics(cfg: shhd.SolverConfig, seed: int):
     return ics
 
 
+def quiescent_ics():
+    """Start of an ensemble realization, evaluated on the device: uniform
+    density 1 and psi = 0 (`set order_parameter 0 equilibrium/gaussian psi0 0
+    kBT 0`); the noise drives it from there."""
+    IC = shhd.initial_condition
+    return [IC(shhd.IC_EQUILIBRIUM_DENSITY, kBT=0.0, value=1.0, pressure_coefficient=1.0),
+            IC(shhd.IC_EQUILIBRIUM_ORDER_PARAMETER, component=0, kBT=0.0, value=0.0, k0=1.0)]
+
+
 def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512), seed: int = 7,
-             low_modes: int = 0) -> List[Job]:
-    """Independent quiescent realizations of uneven sizes, all present at t = 0;
-    with low_modes > 0 each records its lowest Fourier modes of psi every step
-    (FP64 probe rows, for the time correlation)."""
+             low_modes: int = 0, random_start: bool = False) -> List[Job]:
+    """Independent quiescent realizations of uneven sizes, all present at t = 0,
+    each from psi = 0 set on the device (no host state), or with random_start
+    from a long-wavelength random psi built on the host and uploaded at
+    admission; with low_modes > 0 each records its lowest Fourier modes of psi
+    every step (FP64 probe rows, for the time correlation)."""
     import measures
     rng = np.random.default_rng(seed)
     jobs = []
@@ -275,6 +286,11 @@ def ensemble(ninst: int, nsteps: int, seed0: int = 100, sizes=(64, 128, 256, 512
         probes = None
         if low_modes:
             probes = [(shhd.PROBE_STATE, 1, gx, gy) for gx, gy in measures.lowest_modes(cfg, low_modes)]
-        jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps, probes=probes,
-                        make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed), 0.1)))
+        if random_start:
+            jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps, probes=probes,
+                            make_state=lambda cfg=cfg, sseed=sseed: random_state(cfg, np.random.default_rng(sseed),
+                                                                                 0.1)))
+        else:
+            jobs.append(Job(job_id=i, arrival=0.0, cfg=cfg, nsteps=nsteps, probes=probes, make_state=None,
+                            ics=quiescent_ics()))
     return jobs

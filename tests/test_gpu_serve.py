@@ -40,9 +40,10 @@ def test_served_stream_results_equal_alone():
         assert same_result(j, a), f"job {j.job_id}: mixed result differs from alone"
 
 
-def test_ensemble_uneven_sizes():
-    print("[case] ensemble of 12 uneven quiescent instances", flush=True)
-    jobs = serve.ensemble(12, nsteps=9, sizes=(32, 64, 128, 256))
+@pytest.mark.parametrize("random_start", [False, True])
+def test_ensemble_uneven_sizes(random_start):
+    print(f"[case] ensemble of 12 uneven quiescent instances random_start={random_start}", flush=True)
+    jobs = serve.ensemble(12, nsteps=9, sizes=(32, 64, 128, 256), random_start=random_start)
     srv = serve.JobServer(max_chunk=9)
     try:
         rep = srv.run(jobs)
@@ -52,3 +53,4 @@ def test_ensemble_uneven_sizes():
     _, alone = serve.run_sequential(jobs)
     for j, a in zip(jobs, alone):
         assert same_result(j, a), f"job {j.job_id}: mixed result differs from alone"
+        assert np.linalg.norm(a[1]) > 0, "the noise moves psi"
