@@ -371,6 +371,9 @@ struct Instance {
     void* noise = nullptr;    // SRK3 noise of the current step (NoiseMode)
     void* twx = nullptr;
     void* twy = nullptr;
+    // state, k0, tbuf, ubuf, noise and the twiddle tables above are carved out
+    // of this one allocation (one hipMalloc and one hipFree per instance)
+    void* slab = nullptr;
     std::vector<int> static_fields;
     bool is_static[kMaxFields] = {};
     bool rho0_needed = false;
@@ -422,15 +425,7 @@ struct Instance {
         free_probes();
         free_sk();
         if (flux) (void)hipFree(flux);
-        for (int b = 0; b < 3; ++b) {
-            if (state[b]) (void)hipFree(state[b]);
-            if (k0[b]) (void)hipFree(k0[b]);
-        }
-        if (tbuf) (void)hipFree(tbuf);
-        if (ubuf) (void)hipFree(ubuf);
-        if (noise) (void)hipFree(noise);
-        if (twx) (void)hipFree(twx);
-        if (twy) (void)hipFree(twy);
+        if (slab) (void)hipFree(slab);
     }
 };
 
@@ -927,16 +922,28 @@ size_t noise_elems(const Instance& h) {
 }
 
 // bytes a configuration holds on the device (state x3, work planes, noise, twiddles)
-size_t footprint_of(const Instance& h) {
+// The parts of an instance's one device allocation, 256-byte aligned: state
+// x3, FP64 k = 0 copies x3 (FP32 form), T planes, U planes, noise cache,
+// x and y twiddle tables.
+struct SlabParts {
+    size_t state, k0, t, u, noise, twx, twy;
+    size_t total() const { return 3 * state + 3 * k0 + t + u + noise + twx + twy; }
+};
+SlabParts slab_parts(const Instance& h) {
     const size_t nf = static_cast<size_t>(h.desc.nfields);
-    size_t b = 3 * nf * h.field_elems * h.elem;
-    if (h.fp32) b += 3 * nf * sizeof(cx<double>);
-    b += std::max<size_t>(1, static_cast<size_t>(h.n_in)) * h.plane_elems * h.elem;
-    b += std::max<size_t>(1, static_cast<size_t>(h.n_out)) * h.plane_elems * h.elem;
-    b += static_cast<size_t>(h.cfg.nx + h.cfg.ny) * h.elem;
-    b += noise_elems(h) * h.elem;
-    return b;
+    auto al = [](size_t n) { return (n + 255) & ~static_cast<size_t>(255); };
+    SlabParts s;
+    s.state = al(nf * h.field_elems * h.elem);
+    s.k0 = h.fp32 ? al(nf * sizeof(cx<double>)) : 0;
+    s.t = al(std::max<size_t>(1, static_cast<size_t>(h.n_in)) * h.plane_elems * h.elem);
+    s.u = al(std::max<size_t>(1, static_cast<size_t>(h.n_out)) * h.plane_elems * h.elem);
+    s.noise = al(noise_elems(h) * h.elem);
+    s.twx = al(static_cast<size_t>(fz_tw_size(h.cfg.nx)) * h.elem);
+    s.twy = al(static_cast<size_t>(fz_tw_size(h.cfg.ny)) * h.elem);
+    return s;
 }
+
+size_t footprint_of(const Instance& h) { return slab_parts(h).total(); }
 
 // Lower and plan an instance (no device memory yet).
 shhd_status prepare(const shhd_config& cfg, bool batch_shape, std::unique_ptr<Instance>& out,
@@ -969,26 +976,32 @@ shhd_status upload_twiddles(Instance* h, std::string* err) {
     std::vector<cx<T>> tx, ty;
     make_twiddles<T>(h->cfg.nx, tx);
     make_twiddles<T>(h->cfg.ny, ty);
-    SHHD_HIP(err, hipMalloc(&h->twx, tx.size() * sizeof(cx<T>)));
-    SHHD_HIP(err, hipMalloc(&h->twy, ty.size() * sizeof(cx<T>)));
     SHHD_HIP(err, hipMemcpy(h->twx, tx.data(), tx.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
     SHHD_HIP(err, hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
     return SHHD_OK;
 }
 
+// One device allocation per instance, carved into 256-byte aligned parts:
+// state x3 and the FP64 k = 0 copies first (cleared with one memset), then
+// the T and U planes, the noise cache and the two twiddle tables.
 shhd_status allocate(Instance* h, std::string* err) {
-    const size_t nf = static_cast<size_t>(h->desc.nfields);
-    for (int b = 0; b < 3; ++b) {
-        SHHD_HIP(err, hipMalloc(&h->state[b], nf * h->field_elems * h->elem));
-        SHHD_HIP(err, hipMemset(h->state[b], 0, nf * h->field_elems * h->elem));
-        if (h->fp32) {
-            SHHD_HIP(err, hipMalloc(&h->k0[b], nf * sizeof(cx<double>)));
-            SHHD_HIP(err, hipMemset(h->k0[b], 0, nf * sizeof(cx<double>)));
-        }
-    }
-    SHHD_HIP(err, hipMalloc(&h->tbuf, std::max<size_t>(1, static_cast<size_t>(h->n_in)) * h->plane_elems * h->elem));
-    SHHD_HIP(err, hipMalloc(&h->ubuf, std::max<size_t>(1, static_cast<size_t>(h->n_out)) * h->plane_elems * h->elem));
-    if (const size_t ne = noise_elems(*h)) SHHD_HIP(err, hipMalloc(&h->noise, ne * h->elem));
+    const SlabParts parts = slab_parts(*h);
+    const size_t sb = parts.state, kb = parts.k0, tb = parts.t, ub = parts.u, nb = parts.noise, xb = parts.twx;
+    SHHD_HIP(err, hipMalloc(&h->slab, parts.total()));
+    char* p = static_cast<char*>(h->slab);
+    for (int b = 0; b < 3; ++b, p += sb) h->state[b] = p;
+    if (h->fp32)
+        for (int b = 0; b < 3; ++b, p += kb) h->k0[b] = reinterpret_cast<cx<double>*>(p);
+    SHHD_HIP(err, hipMemset(h->slab, 0, 3 * sb + 3 * kb));
+    h->tbuf = p;
+    p += tb;
+    h->ubuf = p;
+    p += ub;
+    if (nb) h->noise = p;
+    p += nb;
+    h->twx = p;
+    p += xb;
+    h->twy = p;
     return h->fp32 ? upload_twiddles<float>(h, err) : upload_twiddles<double>(h, err);
 }
 
