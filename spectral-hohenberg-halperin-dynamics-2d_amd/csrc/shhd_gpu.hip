@@ -1157,6 +1157,10 @@ struct Engine {
     int* d_flags = nullptr;
     std::vector<LaunchClass> classes;
     bool dirty = true;
+    // steps taken since the launch tables were built, mod 3: the tables hold
+    // the three buffer rotations from the one at build time, so a step at
+    // rotation r uses entry r and no call needs a rebuild for its step count
+    int rot = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 
@@ -1807,6 +1811,7 @@ struct Engine {
     shhd_status rebuild() {
         SHHD_HIP(&err, hipStreamSynchronize(stream));
         drop_graph();
+        rot = 0;
         for (auto& c : classes) c.release();
         classes.clear();
         std::map<ClassKey, std::vector<int>> groups;
@@ -1986,6 +1991,12 @@ struct Engine {
         hipLaunchKernelGGL(k_clear_flags, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_flags, kMaxSlots);
         SHHD_HIP(&err, hipGetLastError());
         int done = 0;
+        auto eager = [&]() {
+            const shhd_status st = one_step(rot, false);
+            rot = (rot + 1) % 3;
+            ++done;
+            return st;
+        };
         if (use_graph && !timing && nsteps >= 3) {
             if (!graph_exec) {
                 SHHD_HIP(&err, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -2001,10 +2012,15 @@ struct Engine {
                 graph = g;
                 SHHD_HIP(&err, hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
             }
+            // the graph holds rotations 0, 1, 2: single steps up to rotation 0 first
+            while (rot != 0 && done < nsteps) {
+                const shhd_status st = eager();
+                if (st != SHHD_OK) return st;
+            }
             for (; done + 3 <= nsteps; done += 3) SHHD_HIP(&err, hipGraphLaunch(graph_exec, stream));
         }
-        for (int k = 0; done < nsteps; ++done, ++k) {
-            shhd_status st = one_step(k % 3, false);
+        while (done < nsteps) {
+            const shhd_status st = eager();
             if (st != SHHD_OK) return st;
         }
         for (auto& sp : slots) {
@@ -2013,7 +2029,6 @@ struct Engine {
             sp->step_index += static_cast<uint64_t>(nsteps);
             if (sp->capacity) sp->probe_rows += nsteps;
         }
-        if (nsteps % 3 != 0) dirty = true;   // stage-pointer tables are built for the current rotation
         std::vector<int> flags(kMaxSlots);
         SHHD_HIP(&err, hipMemcpyAsync(flags.data(), d_flags, kMaxSlots * sizeof(int), hipMemcpyDeviceToHost, stream));
         SHHD_HIP(&err, hipStreamSynchronize(stream));
