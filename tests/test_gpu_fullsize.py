@@ -5,6 +5,8 @@
   the reference's mt19937_64 stream, the device a Philox stream): FP64 to
   1e-10 and the FP32 form to 1e-5 relative L2 per field, with the autotuned
   and with the untuned launch shapes (fused 1536-point transforms).
+- the incompressible-turbulence and quiescent models at 1024^2 against the
+  oracle in the same way (the row pass's other two fixed field layouts);
 - Model-H N=2 and incompressible turbulence at 4096^2 (3/2 -> 6144^2), where
   the oracle needs about a minute a step: the fused 6144-point transforms
   against the runtime-plan transforms (the length-generic build the oracle
@@ -57,6 +59,14 @@ def synthetic_state(cfg, seed=2):
         st[f] = np.where(mask, z, 0) * (amp * nc / np.sqrt(2.0 * mask.sum()))
     st[:, 0, 0] = 0
     st[0, 0, 0] = nc
+    if cfg.mode == shhd.MODE_INCOMPRESSIBLE:   # transverse momentum
+        kx, ky, _ = shhd.wavenumbers(cfg)
+        k2 = kx ** 2 + ky ** 2
+        n = cfg.num_order_parameters
+        with np.errstate(invalid="ignore", divide="ignore"):
+            tr = np.where(k2 > 0, (-ky * st[n + 1] + kx * st[n + 2]) / k2, 0)
+        st[n + 1] = -ky * tr
+        st[n + 2] = kx * tr
     return shhd.hermitian_project(st, cfg)
 
 
@@ -113,6 +123,41 @@ def test_model_h_1024_against_oracle(oracle_1024, precision, shapes, tol):
     for f, e in enumerate(errs):
         assert e <= tol, (f, e)
     check_properties(cfg, st, got, 1e-12 if precision == 64 else 1e-6)
+
+
+def other_layout_1024(name, precision):
+    """The incompressible-turbulence and quiescent-ensemble models at 1024^2,
+    noise off: the row pass's other two fixed field layouts (jx, jy; psi)."""
+    if name == "incompressible":
+        return shhd.SolverConfig(nx_active=1024, ny_active=1024, dealias="3/2", dt=1e-4,
+                                 mode=shhd.MODE_INCOMPRESSIBLE, eta=1e-4, momentum_advection=True,
+                                 noise_momentum=False, kBT=0.0, seed=3, precision=precision)
+    return shhd.SolverConfig(nx_active=1024, ny_active=1024, dealias="3/2", lx=1536.0, ly=1536.0, dt=0.02,
+                             mode=shhd.MODE_QUIESCENT, num_order_parameters=1, mobility=[1.0],
+                             free_energy_kind=shhd.FREE_ENERGY_PHI4, mu_k0=[-0.5], mu_k2=[1.0], phi4_u=[1.0],
+                             noise_order_parameter=False, kBT=0.0, seed=5, precision=precision)
+
+
+@pytest.mark.parametrize("name", ["incompressible", "quiescent"])
+def test_other_layouts_1024_against_oracle(name):
+    banner(f"{name} 1024^2 vs oracle, fp64 and fp32, untuned shapes")
+    cfg64 = other_layout_1024(name, 64)
+    st = synthetic_state(cfg64, seed=4)
+    o = op.Oracle(cfg64)
+    o.set_state(st)
+    o.step(3, 0.0)
+    ref = o.get_state()
+    for precision, tol in ((64, 1e-10), (32, 1e-5)):
+        cfg = other_layout_1024(name, precision)
+        _, got, info = run_gpu(cfg, (0, 0, 0, 4, 4, 2), state=st, steps=3)
+        # the fused row pass of this layout and the fused forward columns (the
+        # untuned FP32 inverse-column tile of 4 columns has no fused build)
+        assert info.forward_column_fused > 0 and info.row_fused > 0
+        print(f"    fp{precision} inverse columns fused: {info.inverse_column_fused}", flush=True)
+        errs = [rel_l2(got[f], ref[f]) for f in range(cfg.num_fields) if np.linalg.norm(ref[f]) > 0]
+        print(f"    fp{precision} per-field rel L2 {['%.2e' % e for e in errs]}", flush=True)
+        assert max(errs) <= tol
+        check_properties(cfg, st, got, 1e-12 if precision == 64 else 1e-6)
 
 
 def model_h_4096_ics():
