@@ -869,6 +869,7 @@ void apply_plans(Instance& h, const Tuning& t) {
     if (fz_nb > 0 && d.my % 3 == 0 && 3 * d.nay <= 2 * d.my) {
         for (ColPlan* p : {&h.cols, &h.cols3}) {
             const int v = 6;
+            auto fused_for = [&](int c) { return (c * d.my) % v == 0 && col_fused_built(c * d.my / v, v, d.my); };
             // a tile that leaves the lanes half empty (C * My < NT * 6) takes
             // the columns that fill them when that build exists
             if (p->C * d.my < p->NT * v && (p->NT * v) % d.my == 0) {
@@ -878,7 +879,17 @@ void apply_plans(Instance& h, const Tuning& t) {
                     p->ntiles = (d.hxa + c2 - 1) / c2;
                 }
             }
-            if ((p->C * d.my) % v != 0 || !col_fused_built(p->C * d.my / v, v, d.my)) continue;
+            // an untuned tile with no fused build takes the widest narrower
+            // one that has it (a tuned tile stays as it was measured)
+            if ((p == &h.cols ? t.c1 : t.c3) <= 0 && !fused_for(p->C)) {
+                for (int c2 = p->C >> 1; c2 >= 1; c2 >>= 1) {
+                    if (!fused_for(c2)) continue;
+                    p->C = c2;
+                    p->ntiles = (d.hxa + c2 - 1) / c2;
+                    break;
+                }
+            }
+            if (!fused_for(p->C)) continue;
             const size_t one = static_cast<size_t>(lp_size(p->C * d.my)) * h.elem;
             p->NT = p->C * d.my / v;
             p->V = v;
