@@ -376,45 +376,74 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             // always inside the row (kx = 0 for the zero-padded points and
             // row 0 past the last row), and the padding is zeroed afterwards
             // by selects: a load under a per-point branch is waited for
-            // inside its branch, which serializes the row's loads
+            // inside its branch, which serializes the row's loads.  TH: the
+            // 3/2-padded rows (Nax/2 = FL/3), where each of the lane's R0
+            // points is known at compile time to be padding, kept or
+            // mirrored, except the two runs that straddle FL/3 and 2FL/3.
+            auto load_points = [&]<bool TH>() {
+                constexpr int M = MI::M;
+                constexpr int H = FL / 3;
+                const int hx = TH ? H : hxa;
 #pragma unroll
-            for (int b = 0; b < MI::B; ++b) {
-                const int id = lane + NT * b;
-                if (!MI::valid(id)) continue;
-                const int y = y0 + MI::seq(id);
-                const int j = MI::bfly(id);
-                const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
-#pragma unroll
-                for (int q = 0; q < R0; ++q) {
-                    // one load per field: kx = k (k < Nax/2) or the mirror
-                    // kx = FL - k (k > FL - Nax/2) of the Hermitian row
-                    const int k = j + q * MI::M;
-                    const bool mirror = k > FL - hxa;
-                    const int kk = mirror ? FL - k : (k < hxa ? k : 0);
-                    v[b][q] = gload(ta, row + kk);
-                    vb[b][q] = gload(tb, row + kk);   // tb = ta when there is no second field
+                for (int b = 0; b < MI::B; ++b) {
+                    const int id = lane + NT * b;
+                    if (!MI::valid(id)) continue;
+                    const int y = y0 + MI::seq(id);
+                    const int j = MI::bfly(id);
+                    const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
+                    [&]<int... Q>(std::integer_sequence<int, Q...>) {
+                        ([&] {
+                            constexpr int q = Q;
+                            constexpr bool dead = TH && q * M >= H && q * M + M - 1 <= FL - H;
+                            if constexpr (!dead) {
+                                // one load per field: kx = k (k < Nax/2) or the
+                                // mirror kx = FL - k (k > FL - Nax/2) of the Hermitian row
+                                const int k = j + q * M;
+                                const bool mirror = k > FL - hx;
+                                const int kk = mirror ? FL - k : (k < hx ? k : 0);
+                                v[b][q] = gload(ta, row + kk);
+                                vb[b][q] = gload(tb, row + kk);   // tb = ta when there is no second field
+                            }
+                        }(), ...);
+                    }(std::make_integer_sequence<int, R0>{});
                 }
-            }
 #pragma unroll
-            for (int b = 0; b < MI::B; ++b) {
-                const int id = lane + NT * b;
-                if (!MI::valid(id)) continue;
-                const bool row_ok = y0 + MI::seq(id) < my;
-                const int j = MI::bfly(id);
-#pragma unroll
-                for (int q = 0; q < R0; ++q) {
-                    const int k = j + q * MI::M;
-                    const bool mirror = k > FL - hxa;
-                    const bool live = row_ok && (k < hxa || mirror);
-                    cx<T> a = v[b][q];
-                    cx<T> bb = has_b ? vb[b][q] : czero<T>();
-                    // the kx = 0 mode is real; the mirror half is conj(A) + i conj(B)
-                    a.im = k == 0 ? T(0) : (mirror ? -a.im : a.im);
-                    bb.im = k == 0 ? T(0) : (mirror ? -bb.im : bb.im);
-                    const cx<T> z = mk(a.re - bb.im, a.im + bb.re);   // A + iB
-                    v[b][q] = live ? z : czero<T>();
+                for (int b = 0; b < MI::B; ++b) {
+                    const int id = lane + NT * b;
+                    if (!MI::valid(id)) continue;
+                    const bool row_ok = y0 + MI::seq(id) < my;
+                    const int j = MI::bfly(id);
+                    [&]<int... Q>(std::integer_sequence<int, Q...>) {
+                        ([&] {
+                            constexpr int q = Q;
+                            constexpr bool dead = TH && q * M >= H && q * M + M - 1 <= FL - H;
+                            constexpr bool kept = TH && q * M + M - 1 < H;           // all k < Nax/2
+                            constexpr bool mirrored = TH && q * M > FL - H;          // all k > FL - Nax/2
+                            if constexpr (dead) {
+                                v[b][q] = czero<T>();
+                            } else {
+                                const int k = j + q * M;
+                                const bool mirror = mirrored || (!kept && k > FL - hx);
+                                const bool live = row_ok && (kept || mirrored || k < hx || mirror);
+                                cx<T> a = v[b][q];
+                                cx<T> bb = has_b ? vb[b][q] : czero<T>();
+                                // the kx = 0 mode is real; the mirror half is conj(A) + i conj(B)
+                                if constexpr (q == 0) {
+                                    a.im = k == 0 ? T(0) : (mirror ? -a.im : a.im);
+                                    bb.im = k == 0 ? T(0) : (mirror ? -bb.im : bb.im);
+                                } else {
+                                    a.im = mirror ? -a.im : a.im;
+                                    bb.im = mirror ? -bb.im : bb.im;
+                                }
+                                const cx<T> z = mk(a.re - bb.im, a.im + bb.re);   // A + iB
+                                v[b][q] = live ? z : czero<T>();
+                            }
+                        }(), ...);
+                    }(std::make_integer_sequence<int, R0>{});
                 }
-            }
+            };
+            if (3 * hxa == FL) load_points.template operator()<true>();
+            else load_points.template operator()<false>();
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
             fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
         };
