@@ -149,7 +149,10 @@ shhd_status shhd_gpu_set_state(shhd_gpu* h, const double* interleaved, size_t n)
 shhd_status shhd_gpu_get_state(shhd_gpu* h, double* out, size_t n);
 
 /* nsteps calls of FluidSolver::step(state, t0 + i*dt) (src/project.h:128-145,
- * src/fluid_solver.h:97-99).  Returns after the device work has finished. */
+ * src/fluid_solver.h:97-99).  Returns after the device work has finished.
+ * When the density reaches zero (src/fcalculator_workspace.cc:126-128) the
+ * call stops at that step and returns SHHD_ERR_ZERO_DENSITY with the state as
+ * of the failing step's start, as the reference's throw leaves it. */
 shhd_status shhd_gpu_step(shhd_gpu* h, int nsteps, double t0);
 
 /* FluidSolver::set_flux_request (src/fluid_solver.h:101-103, FluxRequest of
@@ -334,8 +337,15 @@ shhd_status shhd_batch_footprint(const shhd_config* config, size_t* bytes);
  * field zero, as a new reference State, for shhd_batch_apply_initial_condition);
  * SHHD_ERR_FULL when the budget or the slots are used up */
 shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const double* state, size_t n, int* slot);
-/* advance every admitted instance by nsteps */
+/* advance every admitted instance by nsteps.  An instance whose density
+ * reaches zero stops at the failing step's start (as shhd_gpu_step) and takes
+ * no further step until retired; the others advance all nsteps.  The call
+ * returns SHHD_ERR_ZERO_DENSITY when an instance stopped during it. */
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps);
+/* instances stopped by zero density: up to `capacity` slots and, for each,
+ * the 0-based index of the step that failed (= steps completed); *count is
+ * the total number */
+shhd_status shhd_batch_failures(const shhd_batch* b, int* slots, long long* steps, int capacity, int* count);
 shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n);
 /* flux recording of one instance, as shhd_gpu_set_flux_request / shhd_gpu_get_flux */
 shhd_status shhd_batch_set_flux_request(shhd_batch* b, int slot, int density, int order_parameter, int momentum);

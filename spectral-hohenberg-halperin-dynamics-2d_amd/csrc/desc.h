@@ -184,6 +184,14 @@ struct InstDev {
     int ntiles3;              // forward-column tiles
 };
 
+// An instance whose density reached zero stops at the last step it completed
+// (the reference throws inside the failing right-hand side,
+// src/fcalculator_workspace.cc:126-128): every later kernel of that instance
+// returns at entry, so its state buffers keep the failing step's start.
+__device__ __forceinline__ bool instance_failed(const int* flag) {
+    return flag != nullptr && *reinterpret_cast<const volatile int*>(flag) != 0;
+}
+
 struct BlockMap {
     const int* inst;          // instance of each workgroup of the launch
     const int* first;         // first workgroup of each instance

@@ -75,6 +75,7 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
     const int inst = map.inst[blk];
     const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
+    if (instance_failed(I.flag)) return;
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
     const int C = I.C;
@@ -333,6 +334,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
     const int inst = map.inst[blk];
     const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
+    if (instance_failed(I.flag)) return;
     const Desc& d = I.d;
     const cx<T>* __restrict__ tbuf = I.tbuf;
     cx<T>* __restrict__ ubuf = I.ubuf;
@@ -776,6 +778,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     const int inst = map.inst[blk];
     const int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
+    if (instance_failed(I.flag)) return;
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
     const int C = I.C3;

@@ -42,6 +42,7 @@ struct MeasDev {
     unsigned long long* sk_count;
     // common
     const uint64_t* step;       // steps completed before the current one
+    const int* flag;            // zero-density flag: set, the instance took no step
     size_t field_elems;
     unsigned static_mask;       // fields stored once (buffer 0) whatever the rotation
     const cx<double>* k0_static;   // FP32 form: FP64 k = 0 modes of buffer 0, else null
@@ -59,7 +60,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_probe(const MeasDev<T>* __restrict__ meas,
                                                const StagePtrs<T>* __restrict__ last) {
     const MeasDev<T>& M = meas[blockIdx.x];
-    if (M.nprobe == 0) return;
+    if (M.nprobe == 0 || instance_failed(M.flag)) return;
     const StagePtrs<T>& P = last[blockIdx.x];
     const int row = *M.rows;
     if (row < M.capacity) {
@@ -90,7 +91,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_structure_factor(const MeasDev<T>* __restrict__ meas,
                                                           const StagePtrs<T>* __restrict__ last) {
     const MeasDev<T>& M = meas[blockIdx.y];
-    if (M.sk_nf == 0) return;
+    if (M.sk_nf == 0 || instance_failed(M.flag)) return;
     const uint64_t done = *M.step + 1 - M.sk_base;   // steps since accumulation started, this one included
     if (done % static_cast<uint64_t>(M.sk_nevery) != 0) return;
     const StagePtrs<T>& P = last[blockIdx.y];

@@ -266,9 +266,11 @@ __global__ void k_clear_flags(int* flags, int n) {
     if (i < n) flags[i] = 0;
 }
 
-__global__ void k_advance(uint64_t* steps, int n) {
+// step counters of the instances that completed the step (a zero-density
+// flag stops the instance at the failing step)
+__global__ void k_advance(uint64_t* steps, const int* flags, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) steps[i] += 1;
+    if (i < n && flags[i] == 0) steps[i] += 1;
 }
 
 // kx = 0 column of a field that no stage rewrites (rho when incompressible,
@@ -379,6 +381,9 @@ struct Instance {
     bool rho0_needed = false;
     int cur = 0;
     uint64_t step_index = 0;
+    // the density reached zero in step `step_index` (0-based): the state is
+    // that step's start and the instance takes no further step
+    bool failed = false;
     size_t bytes = 0;
     bool batch_shape = false;
     std::string tune_key;
@@ -1162,6 +1167,9 @@ struct Engine {
     hipStream_t stream = nullptr;
     bool batch_shape = false;     // plan small grids onto the shared 256-lane shape
     bool use_graph = false;
+    // batch: an instance whose density reached zero stays stopped until it is
+    // retired; single handle: every step call starts from a cleared flag
+    bool sticky_failures = false;
     size_t budget = 0, used = 0;
     std::vector<std::unique_ptr<Instance>> slots;
     uint64_t* d_steps = nullptr;
@@ -1203,7 +1211,7 @@ struct Engine {
         graph = nullptr;
     }
 
-    shhd_status init(size_t budget_bytes, bool batch, bool graphs) {
+    shhd_status init(size_t budget_bytes, bool batch, bool graphs, bool sticky = false) {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
             return set_err(&err, SHHD_ERR_HIP, "no HIP device available");
@@ -1216,6 +1224,7 @@ struct Engine {
         budget = budget_bytes;
         batch_shape = batch;
         use_graph = graphs;
+        sticky_failures = sticky;
         slots.resize(kMaxSlots);
         return SHHD_OK;
     }
@@ -1244,7 +1253,9 @@ struct Engine {
             if (st != SHHD_OK) return st;
         }
         const uint64_t zero = 0;
+        const int ok = 0;
         SHHD_HIP(&err, hipMemcpy(d_steps + slot, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        SHHD_HIP(&err, hipMemcpy(d_flags + slot, &ok, sizeof(ok), hipMemcpyHostToDevice));
         used += h->bytes;
         slots[static_cast<size_t>(slot)] = std::move(h);
         dirty = true;
@@ -1789,6 +1800,7 @@ struct Engine {
             M.sk = h.d_sk;
             M.sk_count = h.d_sk_count;
             M.step = d_steps + slot;
+            M.flag = d_flags + slot;
             M.field_elems = h.field_elems;
             M.static_mask = 0;
             for (int f : h.static_fields) M.static_mask |= 1u << f;
@@ -1828,7 +1840,7 @@ struct Engine {
         std::map<ClassKey, std::vector<int>> groups;
         for (int s = 0; s < kMaxSlots; ++s) {
             const Instance* h = slots[static_cast<size_t>(s)].get();
-            if (!h) continue;
+            if (!h || h->failed) continue;   // a stopped instance is in no launch
             ClassKey k{h->fp32,      h->stages,     h->cols.NT,     h->cols.V,       h->cols3.NT, h->cols3.V,
                        h->cols3.acc, h->rows.NT,    h->rows.V,      h->rows.maxn,    h->rows.fl,  h->cols.fz,
                        h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf, h->rows.layout};
@@ -1957,7 +1969,7 @@ struct Engine {
             shhd_status st = lc.key.fp32 ? launch_measures<float>(lc, k) : launch_measures<double>(lc, k);
             if (st != SHHD_OK) return st;
         }
-        hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, kMaxSlots);
+        hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, d_flags, kMaxSlots);
         SHHD_HIP(&err, hipGetLastError());
         return SHHD_OK;
     }
@@ -1989,6 +2001,14 @@ struct Engine {
                                                          " of " + std::to_string(sp->capacity) +
                                                          " rows recorded; read the probes first");
         }
+        if (!sticky_failures) {
+            for (auto& sp : slots) {
+                if (sp && sp->failed) {
+                    sp->failed = false;   // a new call retries from the failing step's start
+                    dirty = true;
+                }
+            }
+        }
         if (dirty) {
             shhd_status st = rebuild();
             if (st != SHHD_OK) return st;
@@ -1999,8 +2019,10 @@ struct Engine {
                 sp->flux_clear = false;
             }
         }
-        hipLaunchKernelGGL(k_clear_flags, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_flags, kMaxSlots);
-        SHHD_HIP(&err, hipGetLastError());
+        if (!sticky_failures) {
+            hipLaunchKernelGGL(k_clear_flags, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_flags, kMaxSlots);
+            SHHD_HIP(&err, hipGetLastError());
+        }
         int done = 0;
         auto eager = [&]() {
             const shhd_status st = one_step(rot, false);
@@ -2034,15 +2056,32 @@ struct Engine {
             const shhd_status st = eager();
             if (st != SHHD_OK) return st;
         }
-        for (auto& sp : slots) {
-            if (!sp) continue;
-            sp->cur = (sp->cur + nsteps) % 3;
-            sp->step_index += static_cast<uint64_t>(nsteps);
-            if (sp->capacity) sp->probe_rows += nsteps;
-        }
         std::vector<int> flags(kMaxSlots);
+        std::vector<uint64_t> counts(kMaxSlots);
         SHHD_HIP(&err, hipMemcpyAsync(flags.data(), d_flags, kMaxSlots * sizeof(int), hipMemcpyDeviceToHost, stream));
+        SHHD_HIP(&err, hipMemcpyAsync(counts.data(), d_steps, kMaxSlots * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
         SHHD_HIP(&err, hipStreamSynchronize(stream));
+        // An instance whose density reached zero completed only the steps its
+        // device counter shows; its state is the failing step's start, in the
+        // buffer that step read (no stage writes it).
+        std::string failed_now;
+        for (int s = 0; s < kMaxSlots; ++s) {
+            Instance* sp = slots[static_cast<size_t>(s)].get();
+            if (!sp) continue;
+            uint64_t advanced = static_cast<uint64_t>(nsteps);
+            if (flags[static_cast<size_t>(s)]) {
+                advanced = counts[static_cast<size_t>(s)] - sp->step_index;
+                if (!sp->failed) {
+                    failed_now += (failed_now.empty() ? "" : ", ") + std::string("slot ") + std::to_string(s) +
+                                  " at step " + std::to_string(counts[static_cast<size_t>(s)]);
+                    sp->failed = true;
+                    dirty = true;   // the launch tables drop it
+                }
+            }
+            sp->cur = static_cast<int>((static_cast<uint64_t>(sp->cur) + advanced) % 3);
+            sp->step_index += advanced;
+            if (sp->capacity) sp->probe_rows += static_cast<int>(advanced);
+        }
         if (timing) {
             for (auto& u : ev_used) {
                 float ms = 0.f;
@@ -2054,10 +2093,29 @@ struct Engine {
             }
             ev_used.clear();
         }
-        for (int s = 0; s < kMaxSlots; ++s) {
-            if (slots[static_cast<size_t>(s)] && flags[static_cast<size_t>(s)])
-                return set_err(&err, SHHD_ERR_ZERO_DENSITY, "compressible velocity encountered zero density.");
+        if (!failed_now.empty()) {
+            // the reference's message (src/fcalculator_workspace.cc:126-128); a
+            // batch names the stopped instances
+            std::string msg = "compressible velocity encountered zero density.";
+            if (sticky_failures) msg += " (" + failed_now + ")";
+            return set_err(&err, SHHD_ERR_ZERO_DENSITY, msg);
         }
+        return SHHD_OK;
+    }
+
+    // instances stopped by zero density: slot and the 0-based step that failed
+    shhd_status failures(int* out_slots, long long* out_steps, int capacity, int* count) const {
+        int n = 0;
+        for (int s = 0; s < kMaxSlots; ++s) {
+            const Instance* sp = slots[static_cast<size_t>(s)].get();
+            if (!sp || !sp->failed) continue;
+            if (n < capacity) {
+                if (out_slots) out_slots[n] = s;
+                if (out_steps) out_steps[n] = static_cast<long long>(sp->step_index);
+            }
+            ++n;
+        }
+        if (count) *count = n;
         return SHHD_OK;
     }
 };
@@ -2229,8 +2287,10 @@ shhd_status shhd_gpu_create(const shhd_config* config, shhd_gpu** out) {
     shhd_status st = prepare(*config, false, probe);   // configuration errors before any device work
     if (st != SHHD_OK) return st;
     auto h = std::make_unique<shhd_gpu>();
+    // steps of a call are replayed from a captured 3-step hipGraph
+    // (SHHD_USE_GRAPH=0 launches every kernel eagerly instead)
     const char* g = std::getenv("SHHD_USE_GRAPH");
-    st = h->eng.init(0, false, g && std::atoi(g) != 0);
+    st = h->eng.init(0, false, !g || std::atoi(g) != 0);
     if (st != SHHD_OK) return set_err(nullptr, st, h->eng.err);
     st = h->eng.admit(*config, nullptr, 0, &h->slot);
     if (st != SHHD_OK) return set_err(nullptr, st, h->eng.err);
@@ -2463,7 +2523,7 @@ shhd_status shhd_gpu_get_kernel_bytes(const shhd_gpu* h, double* bytes, int ncla
 shhd_status shhd_batch_create(size_t device_budget_bytes, int use_graphs, shhd_batch** out) {
     if (!out) return set_err(nullptr, SHHD_ERR_INVALID, "null argument");
     auto b = std::make_unique<shhd_batch>();
-    shhd_status st = b->eng.init(device_budget_bytes, true, use_graphs != 0);
+    shhd_status st = b->eng.init(device_budget_bytes, true, use_graphs != 0, true);
     if (st != SHHD_OK) return set_err(nullptr, st, b->eng.err);
     *out = b.release();
     return SHHD_OK;
@@ -2486,6 +2546,11 @@ shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const dou
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps) {
     if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
     return b->eng.step(nsteps);
+}
+
+shhd_status shhd_batch_failures(const shhd_batch* b, int* slots, long long* steps, int capacity, int* count) {
+    if (!b || capacity < 0 || !count) return set_err(nullptr, SHHD_ERR_INVALID, "invalid arguments");
+    return b->eng.failures(slots, steps, capacity, count);
 }
 
 shhd_status shhd_batch_get_state(shhd_batch* b, int slot, double* out, size_t n) {

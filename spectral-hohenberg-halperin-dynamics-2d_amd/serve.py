@@ -39,6 +39,8 @@ class Job:
     finished: float = -1.0
     result: Optional[np.ndarray] = None
     probe_rows: Optional[np.ndarray] = None
+    error: Optional[str] = None            # the job stopped (zero density at step `failed_step`)
+    failed_step: int = -1
 
     @property
     def points(self) -> int:
@@ -59,7 +61,7 @@ class ServeReport:
 
     @property
     def point_steps(self) -> float:
-        return float(sum(j.points * j.nsteps for j in self.jobs))
+        return float(sum(j.points * (j.failed_step if j.error else j.nsteps) for j in self.jobs))
 
     def throughput(self) -> float:
         """Mpoint*steps/s over the whole stream (first arrival to last finish)."""
@@ -119,11 +121,28 @@ class JobServer:
                 continue
             chunk = min(self.max_chunk, min(j.remaining for j in resident))
             tb = self.clock()
-            self.batch.step(chunk)
+            stopped = {}
+            try:
+                self.batch.step(chunk)
+            except shhd.ShhdError as e:
+                if e.status != shhd.ERR_ZERO_DENSITY:
+                    raise
+                # the instances whose density reached zero stopped at that step;
+                # every other job advanced the whole chunk
+                stopped = self.batch.failures()
             busy += self.clock() - tb
             chunks += 1
             still = []
             for j in resident:
+                if j.slot in stopped:
+                    j.failed_step = stopped[j.slot]
+                    j.error = f"compressible velocity encountered zero density. (step {j.failed_step})"
+                    if self.keep_results:
+                        j.result = self.batch.get_state(j.slot)
+                    self.batch.retire(j.slot)
+                    j.finished = self.clock() - t0
+                    done.append(j)
+                    continue
                 j.remaining -= chunk
                 if j.remaining == 0:
                     if self.keep_results:

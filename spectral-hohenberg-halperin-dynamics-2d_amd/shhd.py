@@ -32,6 +32,7 @@ FREE_ENERGY_NULL, FREE_ENERGY_QUADRATIC, FREE_ENERGY_PHI4 = 0, 1, 2
 STATUS = {0: "SHHD_OK", 1: "SHHD_ERR_INVALID", 2: "SHHD_ERR_UNSUPPORTED", 3: "SHHD_ERR_HIP",
           4: "SHHD_ERR_ZERO_DENSITY", 5: "SHHD_ERR_RUNTIME", 6: "SHHD_ERR_FULL"}
 ERR_INVALID = 1
+ERR_ZERO_DENSITY = 4
 ERR_FULL = 6
 
 
@@ -298,6 +299,9 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_batch_failures.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.shhd_batch_failures.restype = ctypes.c_int
     lib.shhd_autotune.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.c_int, ctypes.POINTER(ShhdTuning)]
     lib.shhd_autotune.restype = ctypes.c_int
     lib.shhd_set_tuning.argtypes = [ctypes.POINTER(ShhdConfig), ctypes.POINTER(ShhdTuning)]
@@ -580,7 +584,20 @@ class Batch(_Measures):
         return slot.value
 
     def step(self, nsteps: int):
+        """Advance every admitted instance; raises ShhdError(ERR_ZERO_DENSITY)
+        when an instance stopped at zero density during the call (the others
+        advanced all nsteps; see failures())."""
         self._check(self.lib.shhd_batch_step(self.h, int(nsteps)))
+
+    def failures(self):
+        """{slot: 0-based failing step} of the instances stopped by zero density."""
+        n = ctypes.c_int()
+        self._check(self.lib.shhd_batch_failures(self.h, None, None, 0, ctypes.byref(n)))
+        cap = max(n.value, 1)
+        slots = (ctypes.c_int * cap)()
+        steps = (ctypes.c_longlong * cap)()
+        self._check(self.lib.shhd_batch_failures(self.h, slots, steps, cap, ctypes.byref(n)))
+        return {int(slots[i]): int(steps[i]) for i in range(n.value)}
 
     def get_state(self, slot: int) -> np.ndarray:
         cfg = self.cfgs[slot]

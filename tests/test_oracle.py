@@ -290,7 +290,7 @@ def test_equipartition_known_answer():
     z = (rng.normal(size=k2.shape) + 1j * rng.normal(size=k2.shape)) * math.sqrt(var / 2)
     st[1] = z
     st = shhd.hermitian_project(st, cfg)
-    st[1][0, 1:] *= math.sqrt(2.0)   # kx = 0 line: projection halves the variance
+    st[1][1:, 0] *= math.sqrt(2.0)   # kx = 0 column: projection halves the variance
     st[1][0, 0] = 0
     o = op.Oracle(cfg)
     o.set_state(st)
