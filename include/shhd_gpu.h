@@ -357,6 +357,15 @@ shhd_status shhd_batch_set_structure_factor(shhd_batch* b, int slot, unsigned fi
 shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out, size_t n, long long* nsamples);
 /* initial condition of one instance, as shhd_gpu_apply_initial_condition */
 shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic);
+/* No counterpart in the reference: the same initial condition as
+ * shhd_batch_apply_initial_condition, for callers that cannot allocate device
+ * memory (a serving loop seeding admitted jobs between graph replays, code
+ * under allocation bans).  It calls no hipMalloc or hipFree: the velocity
+ * normalization of the random_vorticity and spectrum/k4 styles keeps its
+ * partial sums (one per workgroup, at most 2048 FP64 values, 16 KiB) in a
+ * static device array of the library that every device keeps until the
+ * process ends.  Calls that overlap on one device race on that array. */
+shhd_status shhd_batch_apply_initial_condition_noalloc(shhd_batch* b, int slot, const shhd_initial_condition* ic);
 /* physical fields of one instance, as shhd_gpu_get_physical */
 shhd_status shhd_batch_get_physical(shhd_batch* b, int slot, double* out, size_t n);
 /* free an instance's device memory and slot */

@@ -36,6 +36,22 @@
 
 using namespace shhd;
 
+// ---------------------------------------------------------------------------
+// Scratch of the entry point that allocates nothing
+// (shhd_batch_apply_initial_condition_noalloc).  The velocity normalization of
+// the random_vorticity and spectrum/k4 initial conditions (k_ic_norm) writes
+// one FP64 partial sum per workgroup, at most kIcNormBlocks of them whatever
+// the grid; the allocating entry point takes them from a per-call hipMalloc,
+// the other from this array.  It holds 2048 doubles (16 KiB) on every device
+// the library runs on, until the process ends, and keeps the last call's
+// partials: a call writes every partial it reads before reading it, so
+// nothing is cleared.  Two such calls that overlap on one device race on it.
+// In host code `scratch` is a host-side stand-in, not a device address: its
+// address on each device comes from hipGetSymbolAddress (scratch_address
+// below), and only that address reaches kernels and copies.
+constexpr int kIcNormBlocks = 2048;
+static __device__ __align__(256) double scratch[kIcNormBlocks];
+
 namespace {
 
 std::string g_create_error;
@@ -49,6 +65,11 @@ bool two_buffers_fit(size_t one_buffer_bytes, int lanes) {
     return 2 * one_buffer_bytes <= (lanes >= 1024 ? 160 * 1024 : 80 * 1024);
 }
 constexpr int kMaxSlots = 4096;
+
+// device address of `scratch`, looked up once per device with that device current
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
 
 struct ColPlan {
     int C = 1, NT = 256, V = 6, ntiles = 1;
@@ -317,6 +338,25 @@ shhd_status set_err(std::nullptr_t, shhd_status st, const std::string& msg) {
         if (e_ != hipSuccess)                                                                     \
             return set_err(errp, SHHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// The device address of `scratch` on `device`, which must be the current
+// device (hipGetSymbolAddress answers for the current one).
+shhd_status scratch_address(int device, double** out, std::string* err) {
+    int cur = -1;
+    SHHD_HIP(err, hipGetDevice(&cur));
+    if (cur != device) return set_err(err, SHHD_ERR_HIP, "the instance's device is not the current device");
+    if (device < 0 || device >= kMaxScratchDevices)
+        return set_err(err, SHHD_ERR_HIP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        SHHD_HIP(err, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(err, SHHD_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return SHHD_OK;
+}
 
 template <typename T>
 void make_twiddles(int L, std::vector<cx<T>>& out) {
@@ -1452,7 +1492,9 @@ struct Engine {
     // `set <target> <style>` evaluated on the device into the resident state
     // (src/initial_condition_*.cc); argument checks and messages follow the
     // styles' parsers and apply() functions.
-    shhd_status apply_initial_condition(int slot, const shhd_initial_condition* icp) {
+    // no_alloc: the normalization partials go to `scratch` instead of a
+    // per-call hipMalloc (nothing on this path allocates device memory)
+    shhd_status apply_initial_condition(int slot, const shhd_initial_condition* icp, bool no_alloc = false) {
         Instance* h = at(slot);
         if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
         if (!icp) return set_err(&err, SHHD_ERR_INVALID, "null initial condition");
@@ -1536,7 +1578,7 @@ struct Engine {
         }
         d.rho0 = rho_hat0 / d.grid;
         const size_t total = static_cast<size_t>(d.hxa) * d.nay;
-        const int nblk = static_cast<int>(std::min<size_t>(2048, (total + 255) / 256));
+        const int nblk = static_cast<int>(std::min<size_t>(kIcNormBlocks, (total + 255) / 256));
         if (ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE || ic.kind == SHHD_IC_EQUILIBRIUM_MOMENTUM_INCOMPRESSIBLE) {
             if (ic.kBT != 0.0 && d.rho0 <= 0.0)
                 return set_err(&err, SHHD_ERR_INVALID, std::string("set momentum ") +
@@ -1545,18 +1587,28 @@ struct Engine {
                                " requires positive reference density.");
         }
         if (ic.kind == SHHD_IC_RANDOM_VORTICITY || ic.kind == SHHD_IC_SPECTRUM_K4) {
+            // one partial per workgroup, summed on the host in workgroup order
+            if (nblk > kIcNormBlocks)
+                return set_err(&err, SHHD_ERR_INVALID, "initial-condition normalization needs " + std::to_string(nblk) +
+                                                           " partial sums, more than the " +
+                                                           std::to_string(kIcNormBlocks) + " the scratch holds");
+            double ps[kIcNormBlocks];
             double* partials = nullptr;
-            SHHD_HIP(&err, hipMalloc(&partials, nblk * sizeof(double)));
+            if (no_alloc) {
+                const shhd_status sa = scratch_address(device, &partials, &err);
+                if (sa != SHHD_OK) return sa;
+            } else {
+                SHHD_HIP(&err, hipMalloc(&partials, nblk * sizeof(double)));
+            }
             hipLaunchKernelGGL(k_ic_norm, dim3(nblk), dim3(256), 0, stream, d, partials);
-            std::vector<double> ps(static_cast<size_t>(nblk));
             const hipError_t e1 = hipGetLastError();
-            const hipError_t e2 = hipMemcpyAsync(ps.data(), partials, nblk * sizeof(double), hipMemcpyDeviceToHost, stream);
+            const hipError_t e2 = hipMemcpyAsync(ps, partials, nblk * sizeof(double), hipMemcpyDeviceToHost, stream);
             const hipError_t e3 = hipStreamSynchronize(stream);
-            (void)hipFree(partials);
+            if (!no_alloc) (void)hipFree(partials);
             if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
                 return set_err(&err, SHHD_ERR_HIP, "initial-condition normalization failed");
             double norm = 0.0;
-            for (double p : ps) norm += p;
+            for (int q = 0; q < nblk; ++q) norm += ps[q];
             if (norm == 0.0) return set_err(&err, SHHD_ERR_INVALID, "set momentum random_vorticity generated zero velocity field.");
             if (d.rho0 == 0.0) return set_err(&err, SHHD_ERR_INVALID, "set momentum random_vorticity requires nonzero reference density.");
             d.scale = ic.urms * d.grid / std::sqrt(norm);
@@ -2598,6 +2650,11 @@ shhd_status shhd_batch_get_structure_factor(shhd_batch* b, int slot, double* out
 shhd_status shhd_batch_apply_initial_condition(shhd_batch* b, int slot, const shhd_initial_condition* ic) {
     if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
     return b->eng.apply_initial_condition(slot, ic);
+}
+
+shhd_status shhd_batch_apply_initial_condition_noalloc(shhd_batch* b, int slot, const shhd_initial_condition* ic) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null batch");
+    return b->eng.apply_initial_condition(slot, ic, true);
 }
 
 shhd_status shhd_batch_get_physical(shhd_batch* b, int slot, double* out, size_t n) {

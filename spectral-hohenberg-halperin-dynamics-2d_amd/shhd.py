@@ -328,6 +328,8 @@ def load_library(path: str = LIB_PATH):
         f = getattr(lib, pre + "get_structure_factor")
         f.argtypes = first + [ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]
         f.restype = ctypes.c_int
+    lib.shhd_batch_apply_initial_condition_noalloc.argtypes = [P, ctypes.c_int, ctypes.POINTER(ShhdInitialCondition)]
+    lib.shhd_batch_apply_initial_condition_noalloc.restype = ctypes.c_int
     lib.shhd_batch_set_flux_request.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.shhd_batch_set_flux_request.restype = ctypes.c_int
     lib.shhd_batch_get_flux.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
@@ -544,6 +546,11 @@ class Batch(_Measures):
 
     def apply_initial_condition(self, slot, ic: ShhdInitialCondition):
         self._check(self.lib.shhd_batch_apply_initial_condition(self.h, int(slot), ctypes.byref(ic)))
+
+    def apply_initial_condition_noalloc(self, slot, ic: ShhdInitialCondition):
+        """The same initial condition with no device allocation on the call
+        (shhd_batch_apply_initial_condition_noalloc)."""
+        self._check(self.lib.shhd_batch_apply_initial_condition_noalloc(self.h, int(slot), ctypes.byref(ic)))
 
     def set_probes(self, slot, probes, capacity):
         self._set_probes(slot, probes, capacity)
