@@ -12,12 +12,14 @@ with kBT = 1e-3 (in-kernel Philox), so every stage draws its stochastic terms.
 resident in HBM; `roofline` is the dominant kernel's algorithmic HBM bytes per
 launch over its mean launch time (HIP events on the solver's stream);
 `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same
-workload on the host cores.
+workload on the host cores.  `north_star` measures the same model at 4096^2
+(3/2 -> 6144^2), FP64 and the FP32 form, each with its own roofline.
 """
 import argparse
 import json
 import math
 import os
+import shutil
 import sys
 import time
 
@@ -102,6 +104,12 @@ def device_initial_conditions(name):
     if name == "incompressible_4096":
         return [IC(shhd.IC_EQUILIBRIUM_DENSITY, seed=3, kBT=0.0, value=1.0, pressure_coefficient=1.0)] + \
                [IC(shhd.IC_SPECTRUM_K4, component=d, seed=3, n0=8.0, urms=1.0) for d in (0, 1)]
+    if name == "model_h_4096":
+        # equilibrium fluctuations set on the device (a 4096^2 host state is
+        # slow to build and upload): rho = 1, psi and j Gaussian, seed 2
+        return ([IC(shhd.IC_EQUILIBRIUM_DENSITY, seed=2, kBT=1e-3, value=1.0, pressure_coefficient=1.0)]
+                + [IC(shhd.IC_EQUILIBRIUM_ORDER_PARAMETER, component=a, seed=2, kBT=1e-4, k0=1.0) for a in (0, 1)]
+                + [IC(shhd.IC_EQUILIBRIUM_MOMENTUM_COMPRESSIBLE, component=d, seed=2, kBT=1e-3) for d in (0, 1)])
     return None
 
 
@@ -151,6 +159,47 @@ def time_gpu(cfg, steps, warmup, ics=None):
     return t1 - t0, kt, kb, info
 
 
+def measured(cfg, steps, warmup, ics, workload_key, precision):
+    """Throughput and the dominant kernel's roofline of one workload."""
+    el, kt, kb, info = time_gpu(cfg, steps, warmup, ics)
+    pts = cfg.nx_active * cfg.ny_active
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms, n = kt[dom]
+    achieved = kb[dom] / (ms / 1e3 / max(n, 1)) / 1e9
+    kernels = {k: {"avg_ms": kt[k][0] / max(kt[k][1], 1), "launches": kt[k][1],
+                   "bytes_per_launch": kb[k], "GBs": kb[k] / (kt[k][0] / 1e3 / max(kt[k][1], 1)) / 1e9
+                   if kt[k][1] else None} for k in kt}
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get(workload_key, {}).get(f"fp{precision}", {}).get(dom)
+        except Exception:
+            traffic = None
+    return {"value": pts * steps / el / 1e6, "ms_per_step": el * 1e3 / steps, "info": info,
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+            "kernels": kernels,
+            "step_GBs": sum(kb[k] * kt[k][1] for k in kt) / max(steps, 1) / (el / steps) / 1e9}
+
+
+def north_star(steps, warmup, tune):
+    """Model-H N=2 at 4096^2 (3/2 -> 6144^2), FP64 and the FP32 form: the
+    configuration the path's throughput target is quoted on."""
+    out = {}
+    for prec in (64, 32):
+        cfg, desc = workload("model_h_4096", prec)
+        shapes = shhd.autotune(cfg, steps=2) if tune else None
+        ics = device_initial_conditions("model_h_4096")
+        m = measured(cfg, steps, warmup, ics, "model_h_4096", prec)
+        m.pop("info")
+        out[f"fp{prec}"] = {"workload": desc, "dtype": f"f{prec}", "launch_shapes": shapes, **m}
+    out["fp32"]["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["fp32"]["error_vs_fp64"] = fp32_error(workload("model_h_4096", 64)[0],
+                                              device_initial_conditions("model_h_4096"), steps=10)
+    return out
+
+
 def ensemble_bench(ninst):
     """Batched ensemble (the quiescent-ensemble workload): ninst quiescent phi4 realizations
     with sizes drawn from {64,128,256,512}^2, each from psi = 0 set on the
@@ -161,7 +210,7 @@ def ensemble_bench(ninst):
     import serve
     import measures
     jobs = serve.ensemble(ninst, nsteps=100, low_modes=16)
-    srv = serve.JobServer(max_chunk=100, keep_results=False)
+    srv = serve.JobServer(max_chunk=100, keep_results=False, policy="lockstep")
     try:
         rep = srv.run(jobs)
     finally:
@@ -184,10 +233,11 @@ def served_bench(njobs):
     """Served stream (the mixed served-job workload, scaled down): njobs with Poisson
     arrivals at 20 jobs/s, grids {256,512,1024,2048}^2 (weights .4/.3/.2/.1),
     N in {0,1,2,4}, FP32 or FP64, steps log-uniform in [10, 200] (the stated
-    [1000, 20000] / 100); continuous batching with admission by footprint."""
+    [1000, 20000] / 100); continuous batching with admission by footprint,
+    least remaining work first in 20 ms chunks (the others held resident)."""
     import serve
     jobs = serve.served_stream(njobs, rate=20.0, steps_range=(10, 200))
-    srv = serve.JobServer(max_chunk=32, keep_results=False)
+    srv = serve.JobServer(keep_results=False, policy="srpt")
     try:
         rep = srv.run(jobs)
     finally:
@@ -197,12 +247,39 @@ def served_bench(njobs):
     seq_s, _ = serve.run_sequential(jobs, keep_results=False, durations=durations)   # neither side downloads
     fifo = np.array(serve.fifo_latencies(jobs, durations))
     pts = rep.point_steps
-    return {"jobs": njobs, "arrival_rate_per_s": 20.0, "p50_s": p50, "p99_s": p99,
+    return {"jobs": njobs, "arrival_rate_per_s": 20.0, "policy": "srpt", "p50_s": p50, "p99_s": p99,
             "Mpoint_steps_s": pts / rep.makespan / 1e6, "busy_Mpoint_steps_s": pts / rep.busy / 1e6,
             "sequential_Mpoint_steps_s": pts / seq_s / 1e6, "speedup_vs_sequential": seq_s / rep.busy,
             "sequential_fifo_p50_s": float(np.percentile(fifo, 50)),
             "sequential_fifo_p99_s": float(np.percentile(fifo, 99)),
             "makespan_s": rep.makespan, "chunks": rep.chunks, "peak_resident": rep.peak_resident}
+
+
+def reference_toolchain_probe():
+    """What the reference binary would need on this host (src/fourier_transform.h
+    includes heffte.h; the build links FFTW and MPI): mpirun, fftw3.h, heFFTe."""
+    def header(name):
+        for d in ("/usr/include", "/usr/local/include", "/opt/conda/include", "/opt/rocm/include"):
+            if os.path.exists(os.path.join(d, name)):
+                return os.path.join(d, name)
+        return None
+    found = {"mpirun": shutil.which("mpirun"), "fftw3.h": header("fftw3.h"), "heffte.h": header("heffte.h")}
+    found["reference_buildable"] = all(found[k] for k in ("mpirun", "fftw3.h", "heffte.h"))
+    return found
+
+
+def host_cpus():
+    """Logical CPUs of the host (nproc), the ones this process may run on, and
+    the threads the oracle uses: OMP_NUM_THREADS when set (the host's CPU share
+    for this job), else every CPU the process may run on."""
+    total = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except Exception:
+        allowed = total
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else allowed
+    return total, allowed, min(threads, allowed)
 
 
 def cpu_baseline(cfg_gpu, budget_s):
@@ -211,13 +288,8 @@ def cpu_baseline(cfg_gpu, budget_s):
     steps until the time budget is spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as op
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        pass
-    cores = min(cores, 16)
-    used = op.set_threads(cores)
+    nproc, allowed, threads = host_cpus()
+    used = op.set_threads(threads)
     cfg = shhd.SolverConfig(**{**cfg_gpu.__dict__, "precision": 64})
     o = op.Oracle(cfg)
     o.set_state(initial_state(cfg))
@@ -233,7 +305,9 @@ def cpu_baseline(cfg_gpu, budget_s):
     pts = cfg.nx_active * cfg.ny_active
     return {"value": pts * n / el / 1e6, "unit": "Mpoint*SRK3-steps/s", "cores": used, "kind": "port",
             "sample": f"{n} SRK3 steps of the same workload after 1 untimed step, {el:.1f} s, "
-                      f"FP64 oracle restatement, {used} OpenMP threads"}
+                      f"FP64 oracle restatement, {used} OpenMP threads",
+            "host": {"nproc": nproc, "cpus_allowed": allowed, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
+            "reference_toolchain": reference_toolchain_probe()}
 
 
 def main():
@@ -253,17 +327,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
-                    help="only run a device-to-device copy of MB megabytes (PMC byte-counter calibration)")
+    ap.add_argument("--no-north-star", action="store_true", help="skip the 4096^2 FP64/FP32 measurement")
+    ap.add_argument("--north-star-steps", type=int, default=60)
     args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        x = torch.ones(args.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
-        for _ in range(3):
-            y = x.clone()
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": x.numel() * 8, "ok": bool(y[0].item() == 1.0)}), flush=True)
-        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration of this path fits one MI355X and runs are "
               "single-GPU only; multi-GPU runs would shard by instance.", file=sys.stderr)
@@ -280,26 +346,10 @@ def main():
     else:
         tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
     ics = device_initial_conditions(args.workload)
-    el, kt, kb, info = time_gpu(cfg, args.steps, args.warmup, ics)
+    m = measured(cfg, args.steps, args.warmup, ics, args.workload, args.precision)
+    info = m["info"]
+    value = m["value"]
     pts = cfg.nx_active * cfg.ny_active
-    value = pts * args.steps / el / 1e6
-
-    # dominant kernel class and its roofline
-    dom = max(kt, key=lambda k: kt[k][0])
-    ms, n = kt[dom]
-    avg_s = ms / 1e3 / max(n, 1)
-    achieved = kb[dom] / avg_s / 1e9
-    kernels = {k: {"avg_ms": kt[k][0] / max(kt[k][1], 1), "launches": kt[k][1],
-                   "bytes_per_launch": kb[k], "GBs": kb[k] / (kt[k][0] / 1e3 / max(kt[k][1], 1)) / 1e9
-                   if kt[k][1] else None} for k in kt}
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(prof):
-        try:
-            t = json.load(open(prof))
-            traffic = t.get(args.workload, {}).get(f"fp{args.precision}", {}).get(dom)
-        except Exception:
-            traffic = None
 
     result = {
         "metric": "Mpoint*SRK3-steps/s",
@@ -308,7 +358,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / args.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -320,18 +370,19 @@ def main():
                    "F_in": info.num_inverse_fields, "F_out": info.num_forward_fields,
                    "F_s": info.num_updated_fields, "parallelism": "single GPU",
                    "launch_shapes": tuning},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "kernels": kernels,
-        "step_GBs": sum(kb[k] * kt[k][1] for k in kt) / max(args.steps, 1) / (el / args.steps) / 1e9,
+        "roofline": m["roofline"],
+        "kernels": m["kernels"],
+        "step_GBs": m["step_GBs"],
     }
     if args.precision == 64 and not args.no_fp32:
         cfg32, _ = workload(args.workload, 32)
         tuning32 = shhd.autotune(cfg32, steps=4) if not args.no_tune else None
-        el32, kt32, kb32, _ = time_gpu(cfg32, args.steps, args.warmup, ics)
-        v32 = pts * args.steps / el32 / 1e6
-        result["fp32"] = {"value": v32, "ms_per_step": el32 * 1e3 / args.steps, "speedup_vs_fp64": v32 / value,
-                          "launch_shapes": tuning32, "error_vs_fp64": fp32_error(cfg, ics)}
+        m32 = measured(cfg32, args.steps, args.warmup, ics, args.workload, 32)
+        result["fp32"] = {"value": m32["value"], "ms_per_step": m32["ms_per_step"],
+                          "speedup_vs_fp64": m32["value"] / value, "launch_shapes": tuning32,
+                          "roofline": m32["roofline"], "error_vs_fp64": fp32_error(cfg, ics)}
+    if not args.no_north_star and args.workload == "model_h_1024" and args.precision == 64:
+        result["north_star"] = north_star(args.north_star_steps, max(2, args.warmup // 4), not args.no_tune)
     if not args.no_serve:
         result["ensemble"] = ensemble_bench(args.ensemble)
         result["served"] = served_bench(args.serve_jobs)

@@ -342,6 +342,10 @@ shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const dou
  * no further step until retired; the others advance all nsteps.  The call
  * returns SHHD_ERR_ZERO_DENSITY when an instance stopped during it. */
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps);
+/* hold = 1: the instance keeps its state and takes no step in later
+ * shhd_batch_step calls until released with hold = 0 (a scheduler runs the
+ * jobs it chooses and leaves the others resident) */
+shhd_status shhd_batch_hold(shhd_batch* b, int slot, int hold);
 /* instances stopped by zero density: up to `capacity` slots and, for each,
  * the 0-based index of the step that failed (= steps completed); *count is
  * the total number */

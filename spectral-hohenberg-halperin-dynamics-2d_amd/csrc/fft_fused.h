@@ -284,4 +284,201 @@ __device__ __forceinline__ void fz_rest_to_regs(
     fz_read<T, NT, L, NSEQ, RL, NSL, fz_pad_after(L, R3F, NP - 2)>(bf.rd(), v, twv, sign, tid);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-local radix-8 / radix-3 pair.  For L = 3M (M a power of two) with the
+// radix-8 pass at ns = L/24 right before the radix-3 pass (radix 3 last), or
+// right after it at ns = 3 (radix 3 first), the two passes exchange data only
+// inside groups of 3 radix-8 and 8 radix-3 butterflies that share one index
+// k < K = L/24.  Lanes are arranged so that a group sits in the four 16-lane
+// rows of one wave at the same position kk: group g = 16 (lane / 64) + kk,
+// sequence s = g / K, k = g mod K, row r = (lane / 16) mod 4.  Row r < 3 does
+// the radix-8 butterfly K r + k (radix 3 last) or 3k + r (radix 3 first);
+// every row does the radix-3 butterflies j3 = k + K (r + 4h), h = 0, 1.  The
+// exchange between the two passes is then a 4 x 4 transpose of (row, value
+// index), done with v_permlane32_swap and v_permlane16_swap (no LDS traffic,
+// no barrier).  NT lanes hold NT / 4 groups = NSEQ K, i.e. NT * 6 = NSEQ L.
+// ---------------------------------------------------------------------------
+template <int L>
+struct FzGrp {
+    static constexpr int K = L / 24;
+    int s, k, r;
+    __device__ __forceinline__ explicit FzGrp(int tid) {
+        const int g = ((tid >> 6) << 4) | (tid & 15);
+        s = g / K;
+        k = g - s * K;
+        r = (tid >> 4) & 3;
+    }
+    // radix-3 butterfly of half h
+    __device__ __forceinline__ int j3(int h) const { return k + K * (r + 4 * h); }
+};
+
+// the pair can replace an LDS exchange for this length and order
+template <int L, bool R3F>
+constexpr bool fz_pair_ok() {
+    if (L % 24 != 0 || fz_npass(L, R3F) < 3) return false;
+    if (R3F) return fz_radix(L, true, 0) == 3 && fz_radix(L, true, 1) == 8;
+    constexpr int NP = fz_npass(L, false);
+    return fz_radix(L, false, NP - 1) == 3 && fz_radix(L, false, NP - 2) == 8 && fz_ns(L, false, NP - 2) == L / 24;
+}
+
+// 4 x 4 transpose of (16-lane row, slot) on 32-bit words: afterwards slot q of
+// row r holds what slot r of row q held.  permlane32_swap(a, b) exchanges the
+// upper half of a's lanes with the lower half of b's (register bit 1 <-> row
+// bit 1), permlane16_swap(a, b) the odd rows of a with the even rows of b
+// (register bit 0 <-> row bit 0).
+__device__ __forceinline__ void fz_xpose4(uint32_t (&x)[4]) {
+    const auto p = __builtin_amdgcn_permlane32_swap(x[0], x[2], false, false);
+    const auto q = __builtin_amdgcn_permlane32_swap(x[1], x[3], false, false);
+    const auto u = __builtin_amdgcn_permlane16_swap(p[0], q[0], false, false);
+    const auto v = __builtin_amdgcn_permlane16_swap(p[1], q[1], false, false);
+    x[0] = u[0];
+    x[1] = u[1];
+    x[2] = v[0];
+    x[3] = v[1];
+}
+template <typename T>
+__device__ __forceinline__ void fz_xpose4(cx<T> (&v)[4]) {
+    constexpr int W = static_cast<int>(sizeof(T) / 4);   // words per real
+    uint32_t x[2 * W][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if constexpr (W == 2) {
+            const uint64_t re = __builtin_bit_cast(uint64_t, v[q].re), im = __builtin_bit_cast(uint64_t, v[q].im);
+            x[0][q] = static_cast<uint32_t>(re);
+            x[1][q] = static_cast<uint32_t>(re >> 32);
+            x[2][q] = static_cast<uint32_t>(im);
+            x[3][q] = static_cast<uint32_t>(im >> 32);
+        } else {
+            x[0][q] = __builtin_bit_cast(uint32_t, v[q].re);
+            x[1][q] = __builtin_bit_cast(uint32_t, v[q].im);
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 2 * W; ++w) fz_xpose4(x[w]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if constexpr (W == 2) {
+            v[q].re = __builtin_bit_cast(double, (static_cast<uint64_t>(x[1][q]) << 32) | x[0][q]);
+            v[q].im = __builtin_bit_cast(double, (static_cast<uint64_t>(x[3][q]) << 32) | x[2][q]);
+        } else {
+            v[q].re = __builtin_bit_cast(float, x[0][q]);
+            v[q].im = __builtin_bit_cast(float, x[1][q]);
+        }
+    }
+}
+
+// Radix 3 last: passes 1 .. npass-3 through LDS, the radix-8 pass at
+// ns = L/24 read from LDS by the group rows, the transpose, and the radix-3
+// pass in registers: w[h][p] = y_s[j3(h) + p L/3].
+template <typename T, int NT, int L, int NSEQ, int NBUF>
+__device__ __forceinline__ void fz_rest_to_regs_pair(FzBufs<T, NBUF>& bf, cx<T> (&w)[2][3],
+                                                     const cx<T>* __restrict__ tw, int sign, int tid) {
+    static_assert(fz_pair_ok<L, false>() && NT * 6 == NSEQ * L, "radix-8 / radix-3 pair shape");
+    constexpr int NP = fz_npass(L, false);
+    constexpr int K = L / 24;
+    constexpr int M8 = L / 8;                       // radix-8 butterflies per sequence
+    constexpr bool PAD = fz_pad_after(L, false, NP - 3);
+    asm volatile("" : "+v"(tid));
+    fz_passes<T, NT, L, NSEQ, false, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 3>{});
+    const FzGrp<L> G(tid);
+    const bool act = G.r < 3;
+    using W8 = std::conditional_t<fz_tw8_planes<T, NT, 8>(), Tw8<T>, Twiddles<T, 8>>;
+    W8 t8;
+    if constexpr (fz_tw8_planes<T, NT, 8>()) t8 = load_tw8<T, K>(tw + fz_tw8_at(L, false, NP - 2), G.k, sign);
+    else t8 = load_twiddles<T, 8>(tw, G.k * (L / (K * 8)), sign);
+    const Twiddles<T, 3> t30 = load_twiddles<T, 3>(tw, G.j3(0), sign);
+    const Twiddles<T, 3> t31 = load_twiddles<T, 3>(tw, G.j3(1), sign);
+    __syncthreads();
+    cx<T> v[8];
+    if (act) {
+        const cx<T>* __restrict__ in = bf.rd();
+        const int a = G.s * L + K * G.r + G.k;
+        if constexpr (PAD && M8 % kPad == 0) {
+            const int a0 = lp(a);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = in[a0 + q * (M8 + M8 / kPad)];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = in[fz_slot<PAD>(a + q * M8)];
+        }
+        if constexpr (fz_tw8_planes<T, NT, 8>()) mul_tw8<T>(v, t8);
+        else mul_twiddles<T, 8>(v, t8);
+        dft_small<T, 8>(v, sign);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = czero<T>();
+    }
+    // row a holds X_a[p], p = 0..7; after the transposes row r holds X_a[r + 4h] in slot a
+    cx<T> lo[4] = {v[0], v[1], v[2], v[3]};
+    cx<T> hi[4] = {v[4], v[5], v[6], v[7]};
+    fz_xpose4<T>(lo);
+    fz_xpose4<T>(hi);
+    cx<T> x0[3] = {lo[0], lo[1], lo[2]};
+    cx<T> x1[3] = {hi[0], hi[1], hi[2]};
+    mul_twiddles<T, 3>(x0, t30);
+    mul_twiddles<T, 3>(x1, t31);
+    dft_small<T, 3>(x0, sign);
+    dft_small<T, 3>(x1, sign);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        w[0][p] = x0[p];
+        w[1][p] = x1[p];
+    }
+}
+
+// Radix 3 first: the radix-3 pass on the lane's inputs v[h][p] =
+// x_s[j3(h) + p L/3], the transpose, and the radix-8 pass at ns = 3 in
+// registers, written to LDS; the remaining passes follow with
+// fz_rest_to_lds_after_pair.
+template <typename T, int NT, int L, int NSEQ, int NBUF>
+__device__ __forceinline__ void fz_first_pair(FzBufs<T, NBUF>& bf, cx<T> (&v)[2][3], const cx<T>* __restrict__ tw,
+                                              int sign, int tid) {
+    static_assert(fz_pair_ok<L, true>() && NT * 6 == NSEQ * L, "radix-3 / radix-8 pair shape");
+    constexpr int K = L / 24;
+    asm volatile("" : "+v"(tid));
+    const FzGrp<L> G(tid);
+    const bool act = G.r < 3;
+    const int k8 = act ? G.r : 0;   // twiddle index of radix-8 butterfly 3k + r (ns = 3)
+    using W8 = std::conditional_t<fz_tw8_planes<T, NT, 8>(), Tw8<T>, Twiddles<T, 8>>;
+    W8 t8;
+    if constexpr (fz_tw8_planes<T, NT, 8>()) t8 = load_tw8<T, 3>(tw + fz_tw8_at(L, true, 1), k8, sign);
+    else t8 = load_twiddles<T, 8>(tw, k8 * (L / 24), sign);
+    dft_small<T, 3>(v[0], sign);
+    dft_small<T, 3>(v[1], sign);
+    // row r holds Z_{r + 4h}[p3] in slot p3; after the transposes row p3 holds
+    // Z_{q}[p3] for q = r + 4h in slot r of half h
+    cx<T> lo[4] = {v[0][0], v[0][1], v[0][2], czero<T>()};
+    cx<T> hi[4] = {v[1][0], v[1][1], v[1][2], czero<T>()};
+    fz_xpose4<T>(lo);
+    fz_xpose4<T>(hi);
+    cx<T> x[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    if (act) {
+        if constexpr (fz_tw8_planes<T, NT, 8>()) mul_tw8<T>(x, t8);
+        else mul_twiddles<T, 8>(x, t8);
+        dft_small<T, 8>(x, sign);
+    }
+    bf.before_write();
+    if (act) {
+        // Stockham output of butterfly j8 = 3k + r at ns = 3: 24 k + r + 3 p
+        cx<T>* __restrict__ out = bf.wr();
+        const int o = G.s * L + 24 * G.k + G.r;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) out[fz_slot<fz_pad_after(L, true, 1)>(o + 3 * p)] = x[p];
+    }
+    bf.flip();
+}
+
+// Passes 2 .. npass-1 through LDS (after fz_first_pair)
+template <typename T, int NT, int L, int NSEQ, int NBUF, int... P>
+__device__ __forceinline__ void fz_passes_from2(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign, int tid,
+                                                std::integer_sequence<int, P...>) {
+    (fz_pass<T, NT, L, NSEQ, true, NBUF, P + 2>(bf, tw, sign, tid), ...);
+}
+template <typename T, int NT, int L, int NSEQ, int NBUF>
+__device__ __forceinline__ void fz_rest_to_lds_after_pair(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign,
+                                                          int tid) {
+    asm volatile("" : "+v"(tid));
+    fz_passes_from2<T, NT, L, NSEQ, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, fz_npass(L, true) - 2>{});
+}
+
 }  // namespace shhd

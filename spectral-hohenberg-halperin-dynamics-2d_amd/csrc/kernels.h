@@ -360,6 +360,12 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         using MI = FzMap<NT, FL, RF, R0>;
         constexpr int B3 = PPT / 3;                 // radix-3 butterflies per lane
         static_assert(FzMap<NT, FL, RF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
+        // FP64: the inverse transforms' last radix-8 / radix-3 passes and the
+        // forward transforms' radix-3 / first radix-8 passes exchange across
+        // lanes (FzGrp map of the lane's six points) where the length allows.
+        // FP32 keeps the LDS exchange: its row pass is bound by VALU issue, and
+        // the pair's fourth wave and transposes add to it (measured slower).
+        constexpr bool PAIR = sizeof(T) == 8 && PPT == 6 && fz_pair_ok<FL, false>() && fz_pair_ok<FL, true>();
         FzBufs<T, NBUF> bf{lds, lds + lp_size(RF * FL), 0};
 
         // ---- inverse x transforms, two fields per complex FFT -------------
@@ -447,7 +453,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             if (3 * hxa == FL) load_points.template operator()<true>();
             else load_points.template operator()<false>();
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
-            fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
+            if constexpr (PAIR) fz_rest_to_regs_pair<T, NT, FL, RF, NBUF>(bf, w, twx, +1, tid);
+            else fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
         };
         if constexpr (LAYOUT > 0) {
             // roles known at compile time: every value goes straight to its array
@@ -490,9 +497,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int id = tid + NT * (i / 3);
-                if (F.make_velocity(i, d.velocity_compressible != 0, r0) &&
-                    y0 + FzMap<NT, FL, RF, 3>::seq(id) < my)
-                    atomicOr(flag, 1);
+                const int row = PAIR ? FzGrp<FL>(tid).s : FzMap<NT, FL, RF, 3>::seq(id);
+                if (F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + row < my) atomicOr(flag, 1);
             }
         }
 
@@ -514,8 +520,13 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             for (int b = 0; b < B3; ++b)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) v[b][p] = mk(va[3 * b + p], vb[3 * b + p]);
-            fz_first<T, NT, FL, RF, true, NBUF>(bf, v, -1, tid);
-            fz_rest_to_lds<T, NT, FL, RF, true, NBUF>(bf, twx, -1, tid);
+            if constexpr (PAIR) {
+                fz_first_pair<T, NT, FL, RF, NBUF>(bf, v, twx, -1, tid);
+                fz_rest_to_lds_after_pair<T, NT, FL, RF, NBUF>(bf, twx, -1, tid);
+            } else {
+                fz_first<T, NT, FL, RF, true, NBUF>(bf, v, -1, tid);
+                fz_rest_to_lds<T, NT, FL, RF, true, NBUF>(bf, twx, -1, tid);
+            }
             __syncthreads();
             const cx<T>* res = bf.rd();
             cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
@@ -901,6 +912,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 2 * B3, "radix-3 last pass, two modes per butterfly");
         static_assert(FzMap<NT, FZL, CF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
         using M0 = FzMap<NT, FZL, CF, R0>;
+        // FP64: the last radix-8 and the radix-3 pass exchange across lanes where
+        // the length allows; the kept modes then follow the FzGrp map (FP32
+        // keeps the LDS exchange, as the row pass does)
+        constexpr bool PAIR = sizeof(T) == 8 && VPT == 6 && fz_pair_ok<FZL, false>();
+        auto seq_of = [&](int b, int lane_) { return PAIR ? FzGrp<FZL>(lane_).s : FzMap<NT, FZL, CF, 3>::seq(lane_ + NT * b); };
+        auto bfly_of = [&](int b, int lane_) {
+            return PAIR ? FzGrp<FZL>(lane_).j3(b) : FzMap<NT, FZL, CF, 3>::bfly(lane_ + NT * b);
+        };
         FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
         // the first butterflies of contribution ci, straight from U
         auto load_contrib = [&](int ci, cx<T> (&v)[M0::B][R0]) {
@@ -929,16 +948,16 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
             // (the barriers wait for LDS operations only)
             if (ci + 1 < g.ncontrib) load_contrib(ci + 1, v);
             cx<T> w[B3][3];
-            fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
+            if constexpr (PAIR) fz_rest_to_regs_pair<T, NT, FZL, CF, NBUF>(bf, w, twy, -1, tid);
+            else fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
             // an opaque copy of the lane index: the mode coordinates below are
             // recomputed per contribution instead of held live across the transform
             int lane = tid;
             asm volatile("" : "+v"(lane));
 #pragma unroll
             for (int b = 0; b < B3; ++b) {
-                const int id = lane + NT * b;
-                const int s = FzMap<NT, FZL, CF, 3>::seq(id);
-                const int j = FzMap<NT, FZL, CF, 3>::bfly(id);
+                const int s = seq_of(b, lane);
+                const int j = bfly_of(b, lane);
                 const int gx = gx0 + s;
                 if (gx >= hxa) continue;
                 if (j < half) accumulate(cf, 2 * b, w[b][0], gx, j, j);
@@ -951,9 +970,8 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < B3; ++b) {
-            const int id = tid + NT * b;
-            const int s = FzMap<NT, FZL, CF, 3>::seq(id);
-            const int j = FzMap<NT, FZL, CF, 3>::bfly(id);
+            const int s = seq_of(b, tid);
+            const int j = bfly_of(b, tid);
             if (gx0 + s >= hxa) continue;
             if (j < half) {
                 acc_x[s * nay + j] = acc0[2 * b];

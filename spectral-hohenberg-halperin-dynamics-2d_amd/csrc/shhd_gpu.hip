@@ -287,11 +287,11 @@ __global__ void k_clear_flags(int* flags, int n) {
     if (i < n) flags[i] = 0;
 }
 
-// step counters of the instances that completed the step (a zero-density
-// flag stops the instance at the failing step)
-__global__ void k_advance(uint64_t* steps, const int* flags, int n) {
+// step counters of the instances that completed the step: those in a launch
+// class (active) whose zero-density flag is clear
+__global__ void k_advance(uint64_t* steps, const int* flags, const int* active, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flags[i] == 0) steps[i] += 1;
+    if (i < n && active[i] != 0 && flags[i] == 0) steps[i] += 1;
 }
 
 // kx = 0 column of a field that no stage rewrites (rho when incompressible,
@@ -424,6 +424,8 @@ struct Instance {
     // the density reached zero in step `step_index` (0-based): the state is
     // that step's start and the instance takes no further step
     bool failed = false;
+    // held by the caller (shhd_batch_hold): keeps its state, takes no step
+    bool held = false;
     size_t bytes = 0;
     bool batch_shape = false;
     std::string tune_key;
@@ -1214,6 +1216,7 @@ struct Engine {
     std::vector<std::unique_ptr<Instance>> slots;
     uint64_t* d_steps = nullptr;
     int* d_flags = nullptr;
+    int* d_active = nullptr;      // 1 for the slots in a launch class (they step)
     std::vector<LaunchClass> classes;
     bool dirty = true;
     // steps taken since the launch tables were built, mod 3: the tables hold
@@ -1236,6 +1239,7 @@ struct Engine {
         slots.clear();
         if (d_steps) (void)hipFree(d_steps);
         if (d_flags) (void)hipFree(d_flags);
+        if (d_active) (void)hipFree(d_active);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto& u : ev_used) {
             (void)hipEventDestroy(u.second.first);
@@ -1261,6 +1265,8 @@ struct Engine {
         SHHD_HIP(&err, hipMalloc(&d_flags, kMaxSlots * sizeof(int)));
         SHHD_HIP(&err, hipMemset(d_steps, 0, kMaxSlots * sizeof(uint64_t)));
         SHHD_HIP(&err, hipMemset(d_flags, 0, kMaxSlots * sizeof(int)));
+        SHHD_HIP(&err, hipMalloc(&d_active, kMaxSlots * sizeof(int)));
+        SHHD_HIP(&err, hipMemset(d_active, 0, kMaxSlots * sizeof(int)));
         budget = budget_bytes;
         batch_shape = batch;
         use_graph = graphs;
@@ -1892,18 +1898,21 @@ struct Engine {
         std::map<ClassKey, std::vector<int>> groups;
         for (int s = 0; s < kMaxSlots; ++s) {
             const Instance* h = slots[static_cast<size_t>(s)].get();
-            if (!h || h->failed) continue;   // a stopped instance is in no launch
+            if (!h || h->failed || h->held) continue;   // a stopped or held instance is in no launch
             ClassKey k{h->fp32,      h->stages,     h->cols.NT,     h->cols.V,       h->cols3.NT, h->cols3.V,
                        h->cols3.acc, h->rows.NT,    h->rows.V,      h->rows.maxn,    h->rows.fl,  h->cols.fz,
                        h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf, h->rows.layout};
             groups[k].push_back(s);
         }
+        std::vector<int> active(kMaxSlots, 0);
         for (auto& g : groups) {
             LaunchClass lc;
             lc.key = g.first;
             lc.members = g.second;
             classes.push_back(lc);
+            for (int s : g.second) active[static_cast<size_t>(s)] = 1;
         }
+        SHHD_HIP(&err, hipMemcpy(d_active, active.data(), kMaxSlots * sizeof(int), hipMemcpyHostToDevice));
         for (auto& lc : classes) {
             shhd_status st = lc.key.fp32 ? build_class_tables<float>(lc) : build_class_tables<double>(lc);
             if (st != SHHD_OK) return st;
@@ -2021,7 +2030,8 @@ struct Engine {
             shhd_status st = lc.key.fp32 ? launch_measures<float>(lc, k) : launch_measures<double>(lc, k);
             if (st != SHHD_OK) return st;
         }
-        hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, d_flags, kMaxSlots);
+        hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, d_flags, d_active,
+                           kMaxSlots);
         SHHD_HIP(&err, hipGetLastError());
         return SHHD_OK;
     }
@@ -2120,7 +2130,7 @@ struct Engine {
         for (int s = 0; s < kMaxSlots; ++s) {
             Instance* sp = slots[static_cast<size_t>(s)].get();
             if (!sp) continue;
-            uint64_t advanced = static_cast<uint64_t>(nsteps);
+            uint64_t advanced = sp->held ? 0 : static_cast<uint64_t>(nsteps);
             if (flags[static_cast<size_t>(s)]) {
                 advanced = counts[static_cast<size_t>(s)] - sp->step_index;
                 if (!sp->failed) {
@@ -2151,6 +2161,17 @@ struct Engine {
             std::string msg = "compressible velocity encountered zero density.";
             if (sticky_failures) msg += " (" + failed_now + ")";
             return set_err(&err, SHHD_ERR_ZERO_DENSITY, msg);
+        }
+        return SHHD_OK;
+    }
+
+    // hold: the instance keeps its state and takes no step until released
+    shhd_status hold(int slot, bool on) {
+        Instance* h = at(slot);
+        if (!h) return set_err(&err, SHHD_ERR_INVALID, "no instance in slot " + std::to_string(slot));
+        if (h->held != on) {
+            h->held = on;
+            dirty = true;
         }
         return SHHD_OK;
     }
@@ -2598,6 +2619,11 @@ shhd_status shhd_batch_admit(shhd_batch* b, const shhd_config* config, const dou
 shhd_status shhd_batch_step(shhd_batch* b, int nsteps) {
     if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null handle");
     return b->eng.step(nsteps);
+}
+
+shhd_status shhd_batch_hold(shhd_batch* b, int slot, int hold) {
+    if (!b) return set_err(nullptr, SHHD_ERR_INVALID, "null batch");
+    return b->eng.hold(slot, hold != 0);
 }
 
 shhd_status shhd_batch_failures(const shhd_batch* b, int* slots, long long* steps, int capacity, int* count) {

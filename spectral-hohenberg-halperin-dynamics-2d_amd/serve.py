@@ -3,10 +3,11 @@
 Jobs (model, grid, steps, seed, precision) arrive over time; the loop keeps
 every admitted job's state resident in HBM, admits queued jobs in arrival
 order whenever the device budget and slots allow (admission by footprint,
-`shhd_batch_footprint`), advances all resident jobs together in chunks of
-steps (one launch per kernel, stage and launch shape, replayed from captured
-hipGraphs), and retires each job as soon as its own step count is reached, so
-jobs that arrive while others run join at the next chunk boundary.
+`shhd_batch_footprint`), advances the resident jobs in time-bounded chunks of
+steps (one launch per kernel, stage and launch shape for every job that
+runs, replayed from captured hipGraphs) with the least remaining work first,
+and retires each job as soon as its own step count is reached, so jobs that
+arrive while others run join at the next chunk boundary.
 
 Each job's result is bit-identical to running it alone (every instance uses
 its own Philox counters and the same kernels whatever else is resident).
@@ -73,21 +74,60 @@ class ServeReport:
 
 
 class JobServer:
+    """Continuous-batching server.
+
+    policy "srpt" (default): every job that has arrived is admitted while the
+    device budget and slots allow, and each chunk steps the resident jobs with
+    the least remaining work (the smallest, every job within twice its
+    remaining work, and further jobs in that order while the chunk's step
+    stays under `fill_s`, so cheap jobs share the launches); the others stay
+    resident but held.  Chunks are bounded by time (`quantum_s`, from an
+    online estimate of the step cost), so new arrivals are ranked at least
+    that often.  A short job that arrives behind a long one therefore finishes
+    at once, and no job waits behind work larger than its own.
+    policy "lockstep": every resident job steps in every chunk (chunk =
+    min(max_chunk, smallest remaining step count)).
+    """
+
     def __init__(self, budget_bytes: int = 0, max_chunk: int = 48, use_graphs: bool = True,
-                 keep_results: bool = True, clock: Callable[[], float] = time.perf_counter):
+                 keep_results: bool = True, clock: Callable[[], float] = time.perf_counter, policy: str = "srpt",
+                 quantum_s: float = 0.02, fill_s: float = 2e-4):
+        if policy not in ("srpt", "lockstep"):
+            raise ValueError(f"unknown policy {policy}")
         self.batch = shhd.Batch(budget_bytes=budget_bytes, use_graphs=use_graphs)
         self.budget = budget_bytes
         self.max_chunk = max_chunk
         self.keep_results = keep_results
         self.clock = clock
+        self.policy = policy
+        self.quantum = quantum_s
+        self.fill = fill_s
+        self.sec_per_unit = 6e-10   # step cost per weighted point (calibrated online)
 
     def close(self):
         self.batch.close()
+
+    @staticmethod
+    def _units(job: "Job") -> float:
+        """Relative cost of one step of a job: active points x fields, FP32 at
+        about 0.6 of FP64 (the measured ratio of the step times)."""
+        c = job.cfg
+        return c.nx_active * c.ny_active * (c.num_order_parameters + 3) / 5.0 * (1.0 if c.precision == 64 else 0.6)
+
+    def _finish(self, j: "Job", t0: float, done: List["Job"]):
+        if self.keep_results:
+            j.result = self.batch.get_state(j.slot)
+        if j.probes:
+            j.probe_rows = self.batch.read_probes(j.slot)
+        self.batch.retire(j.slot)
+        j.finished = self.clock() - t0
+        done.append(j)
 
     def run(self, jobs: List[Job]) -> ServeReport:
         pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
         resident: List[Job] = []
         done: List[Job] = []
+        held = set()
         t0 = self.clock()
         chunks = 0
         peak = 0
@@ -119,7 +159,24 @@ class JobServer:
                 if wait > 0:
                     time.sleep(wait)
                 continue
-            chunk = min(self.max_chunk, min(j.remaining for j in resident))
+            if self.policy == "lockstep":
+                run = resident
+                chunk = min(self.max_chunk, min(j.remaining for j in resident))
+            else:
+                order = sorted(resident, key=lambda j: (j.remaining * self._units(j), j.arrival))
+                least = order[0].remaining * self._units(order[0])
+                run, step_s = [], 0.0
+                for j in order:
+                    cost = self.sec_per_unit * self._units(j)
+                    if not run or j.remaining * self._units(j) <= 2.0 * least or step_s + cost <= self.fill:
+                        run.append(j)
+                        step_s += cost
+                chunk = min(min(j.remaining for j in run), max(1, int(self.quantum / max(step_s, 1e-9))))
+                runs = {j.job_id for j in run}
+                for j in resident:   # hold what does not run, release what does
+                    if (j.job_id in runs) == (j.slot in held):
+                        self.batch.hold(j.slot, j.job_id not in runs)
+                        (held.discard if j.job_id in runs else held.add)(j.slot)
             tb = self.clock()
             stopped = {}
             try:
@@ -130,28 +187,26 @@ class JobServer:
                 # the instances whose density reached zero stopped at that step;
                 # every other job advanced the whole chunk
                 stopped = self.batch.failures()
-            busy += self.clock() - tb
+            el = self.clock() - tb
+            busy += el
             chunks += 1
+            runs = {j.job_id for j in run}
+            units = sum(self._units(j) for j in run) * chunk
+            if units > 0 and not stopped:
+                self.sec_per_unit = 0.7 * self.sec_per_unit + 0.3 * el / units
             still = []
             for j in resident:
                 if j.slot in stopped:
                     j.failed_step = stopped[j.slot]
                     j.error = f"compressible velocity encountered zero density. (step {j.failed_step})"
-                    if self.keep_results:
-                        j.result = self.batch.get_state(j.slot)
-                    self.batch.retire(j.slot)
-                    j.finished = self.clock() - t0
-                    done.append(j)
+                    held.discard(j.slot)
+                    self._finish(j, t0, done)
                     continue
-                j.remaining -= chunk
+                if j.job_id in runs:
+                    j.remaining -= chunk
                 if j.remaining == 0:
-                    if self.keep_results:
-                        j.result = self.batch.get_state(j.slot)
-                    if j.probes:
-                        j.probe_rows = self.batch.read_probes(j.slot)
-                    self.batch.retire(j.slot)
-                    j.finished = self.clock() - t0
-                    done.append(j)
+                    held.discard(j.slot)
+                    self._finish(j, t0, done)
                 else:
                     still.append(j)
             resident = still

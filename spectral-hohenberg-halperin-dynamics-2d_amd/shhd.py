@@ -299,6 +299,8 @@ def load_library(path: str = LIB_PATH):
     lib.shhd_batch_admit.restype = ctypes.c_int
     lib.shhd_batch_step.argtypes = [P, ctypes.c_int]
     lib.shhd_batch_step.restype = ctypes.c_int
+    lib.shhd_batch_hold.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    lib.shhd_batch_hold.restype = ctypes.c_int
     lib.shhd_batch_failures.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.shhd_batch_failures.restype = ctypes.c_int
@@ -595,6 +597,10 @@ class Batch(_Measures):
         when an instance stopped at zero density during the call (the others
         advanced all nsteps; see failures())."""
         self._check(self.lib.shhd_batch_step(self.h, int(nsteps)))
+
+    def hold(self, slot: int, on: bool = True):
+        """Keep an instance resident without stepping it (on=False releases it)."""
+        self._check(self.lib.shhd_batch_hold(self.h, int(slot), int(bool(on))))
 
     def failures(self):
         """{slot: 0-based failing step} of the instances stopped by zero density."""
