@@ -9,7 +9,7 @@
 // inputs x[j + q L/R] twiddled by w_L^{sign q k L/(ns R)}, outputs at
 // (j - k) R + k + p ns), with the sub-length ns no longer required to be a
 // power of two, so the radix-3 pass may come first.  Two radix orders:
-//   R3 last  (8s, 4, 2, then 3): the last pass's butterfly j (j < L/3) ends
+//   R3 last  (4 or 2, 8s, then 3): the last pass's butterfly j (j < L/3) ends
 //            with the outputs j, j + L/3, j + 2L/3;
 //   R3 first (3, then 8s, 4, 2): the first pass's butterfly j takes the
 //            inputs j, j + L/3, j + 2L/3.
@@ -25,9 +25,27 @@
 
 namespace shhd {
 
+// Radix 3 last puts the 4 or 2 that the radix-8 passes leave of L/3 first,
+// so that a radix-8 pass (at ns = L/24) always precedes the radix-3 pass (the
+// pair the cross-lane exchange below merges): 3072 = 2 8 8 8 3, 6144 = 4 8 8 8 3.
 constexpr int fz_radix(int L, bool r3_first, int P) {
-    if (r3_first && L % 3 == 0) return P == 0 ? 3 : ct_radix(L / 3, P - 1);
-    return ct_radix(L, P);
+    if (L % 3 != 0) return ct_radix(L, P);
+    if (r3_first) return P == 0 ? 3 : ct_radix(L / 3, P - 1);
+    int rest = L / 3;   // a power of two for the 3 * 2^k lengths
+    while (rest % 8 == 0) rest /= 8;
+    if (rest != 1 && rest != 2 && rest != 4) return ct_radix(L, P);
+    int p = P;
+    if (rest > 1) {
+        if (p == 0) return rest;
+        --p;
+    }
+    int m = L / 3 / rest;
+    while (m % 8 == 0) {
+        if (p == 0) return 8;
+        m /= 8;
+        --p;
+    }
+    return p == 0 ? 3 : 0;
 }
 constexpr int fz_npass(int L, bool r3_first) {
     int n = 0;

@@ -360,12 +360,14 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         using MI = FzMap<NT, FL, RF, R0>;
         constexpr int B3 = PPT / 3;                 // radix-3 butterflies per lane
         static_assert(FzMap<NT, FL, RF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
-        // FP64: the inverse transforms' last radix-8 / radix-3 passes and the
-        // forward transforms' radix-3 / first radix-8 passes exchange across
-        // lanes (FzGrp map of the lane's six points) where the length allows.
-        // FP32 keeps the LDS exchange: its row pass is bound by VALU issue, and
-        // the pair's fourth wave and transposes add to it (measured slower).
-        constexpr bool PAIR = sizeof(T) == 8 && PPT == 6 && fz_pair_ok<FL, false>() && fz_pair_ok<FL, true>();
+        // The inverse transforms' last radix-8 / radix-3 passes and the forward
+        // transforms' radix-3 / first radix-8 passes exchange across lanes
+        // (FzGrp map of the lane's six points) in FP64 below 1024 lanes.  The
+        // 1024-lane FP64 builds are short of registers (they would spill
+        // more), and FP32 keeps the LDS exchange: its 1536-point row pass is
+        // bound by VALU issue, which the pair's fourth wave and transposes add
+        // to (measured 22 % slower), and at 6144 points it measured 2 % slower.
+        constexpr bool PAIR = sizeof(T) == 8 && NT < 1024 && PPT == 6 && fz_pair_ok<FL, false>() && fz_pair_ok<FL, true>();
         FzBufs<T, NBUF> bf{lds, lds + lp_size(RF * FL), 0};
 
         // ---- inverse x transforms, two fields per complex FFT -------------
@@ -912,10 +914,10 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 2 * B3, "radix-3 last pass, two modes per butterfly");
         static_assert(FzMap<NT, FZL, CF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
         using M0 = FzMap<NT, FZL, CF, R0>;
-        // FP64: the last radix-8 and the radix-3 pass exchange across lanes where
-        // the length allows; the kept modes then follow the FzGrp map (FP32
-        // keeps the LDS exchange, as the row pass does)
-        constexpr bool PAIR = sizeof(T) == 8 && VPT == 6 && fz_pair_ok<FZL, false>();
+        // The last radix-8 and the radix-3 pass exchange across lanes (FP64
+        // below 1024 lanes, as the row pass); the kept modes then follow the
+        // FzGrp map
+        constexpr bool PAIR = sizeof(T) == 8 && NT < 1024 && VPT == 6 && fz_pair_ok<FZL, false>();
         auto seq_of = [&](int b, int lane_) { return PAIR ? FzGrp<FZL>(lane_).s : FzMap<NT, FZL, CF, 3>::seq(lane_ + NT * b); };
         auto bfly_of = [&](int b, int lane_) {
             return PAIR ? FzGrp<FZL>(lane_).j3(b) : FzMap<NT, FZL, CF, 3>::bfly(lane_ + NT * b);

@@ -133,3 +133,25 @@ def test_admission_budget_and_retire():
         b.step(3)
         assert b.usage()[0] == 2
         assert np.all(np.isfinite(b.get_state(s1))) and np.all(np.isfinite(b.get_state(s2)))
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_held_instance_keeps_its_state_and_resumes(graphs):
+    """A held instance takes no step (state, step counter and noise stream
+    unchanged) while the others advance; released, it continues exactly as if
+    it had never been held."""
+    banner(f"held instance resumes, graphs={graphs}")
+    ca, cb = model_h(32, seed=7), quiescent(64, 104)
+    sa, sb = state_for(ca, 11), state_for(cb, 12)
+    ref_a = alone(ca, sa, 8, graphs)
+    ref_b = alone(cb, sb, 13, graphs)
+    with shhd.Batch(use_graphs=graphs) as b:
+        ia, ib = b.admit(ca, sa), b.admit(cb, sb)
+        b.step(3)
+        b.hold(ia)
+        b.step(5)                       # only b advances
+        b.hold(ia, False)
+        b.step(5)
+        got_a, got_b = b.get_state(ia), b.get_state(ib)
+    assert np.array_equal(got_a, ref_a)
+    assert np.array_equal(got_b, ref_b)
