@@ -305,6 +305,12 @@ typedef struct shhd_tuning {
                                   builds with two LDS buffers where two
                                   workgroups still fit a CU (4, 4) and blocks
                                   of four columns (2).                       */
+    int inverse_plane_block;   /* kx blocks of the column-to-row planes (the
+                                  inverse column pass writes them, the row
+                                  pass reads them): 0 row-major (the default),
+                                  k > 0 blocks of 2^(k-1) columns laid out as
+                                  the row-to-column planes' (a block that
+                                  does not divide Nax/2 uses row-major)      */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

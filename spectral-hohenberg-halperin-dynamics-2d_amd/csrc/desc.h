@@ -75,6 +75,8 @@ struct Desc {
     int lnay, lhxa;             // log2 of nay, hxa when powers of two, else -1
     int ulg;                    // U planes in blocks of 2^ulg kx columns: element (kx, y)
                                 // at ((kx >> ulg) My + y) 2^ulg + (kx mod 2^ulg); 2^ulg divides Nax/2
+    int tlg;                    // T planes: -1 row-major (element (kx, y) at y Nax/2 + kx), else
+                                // blocks of 2^tlg kx columns laid out as U's; 2^tlg divides Nax/2
     double lx, ly;
     double dkx, dky;            // 2 pi / Lx, 2 pi / Ly
     int N, nfields, mode;
@@ -168,7 +170,7 @@ struct InstDev {
     const cx<T>* twx;         // twiddles exp(-2 pi i j / Mx), j < Mx, then the
                               // radix-8 rows of the fused transforms (fz_tw8_at)
     const cx<T>* twy;         // same for My
-    cx<T>* tbuf;              // K1 -> K2 planes, row-major: [field][y][kx]
+    cx<T>* tbuf;              // K1 -> K2 planes, [field] of My rows x Nax/2 kx (Desc::tlg)
     cx<T>* ubuf;              // K2 -> K3 planes, kx-blocked column-major (Desc::ulg)
     int* flag;                // zero-density flag
     const uint64_t* step;     // Philox step counter of this instance

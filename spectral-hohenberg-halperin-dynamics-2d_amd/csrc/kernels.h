@@ -51,6 +51,11 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// element (kx, y) of a T plane in the blocked layout (Desc::tlg >= 0)
+__device__ __forceinline__ uint32_t t_at(const Desc& d, int kx, int y) {
+    return ((static_cast<uint32_t>(kx >> d.tlg) * d.my + y) << d.tlg) + (kx & ((1 << d.tlg) - 1));
+}
+
 // v / n for n > 0: a shift when ln = log2(n) >= 0 (n a power of two), else a division
 __device__ __forceinline__ int divn(int v, int n, int ln) { return ln >= 0 ? (v >> ln) : v / n; }
 // log2 of a power of two
@@ -160,11 +165,27 @@ __global__ __launch_bounds__(NT) void k_inverse_columns(const InstDev<T>* __rest
     }
     cx<T>* __restrict__ dst = tbuf + static_cast<size_t>(slot) * L * d.hxa;
     const int lc = log2p2(C);
-    for (int idx = threadIdx.x; idx < ncol; idx += NT) {
-        const int y = idx >> lc;
-        const int c = idx - (y << lc);
-        const int gx = gx0 + c;
-        if (gx < d.hxa) gstore(dst, static_cast<uint32_t>(y * d.hxa + gx), res[lp(c * L + y)]);
+    if (d.tlg < 0) {
+        // row-major T: the lanes of a row write its C columns side by side
+        for (int idx = threadIdx.x; idx < ncol; idx += NT) {
+            const int y = idx >> lc;
+            const int c = idx - (y << lc);
+            const int gx = gx0 + c;
+            if (gx < d.hxa) gstore(dst, static_cast<uint32_t>(y * d.hxa + gx), res[lp(c * L + y)]);
+        }
+    } else {
+        // kx-blocked T: the tile's columns of one block, all rows, are one
+        // contiguous run of the plane (lb = log2 of the columns a run holds)
+        const int lb = d.tlg < lc ? d.tlg : lc;
+        const int run = L << lb;
+        for (int cb = 0; cb < (C >> lb); ++cb) {
+            for (int e = threadIdx.x; e < run; e += NT) {
+                const int y = e >> lb;
+                const int c = (cb << lb) + (e & ((1 << lb) - 1));
+                const int gx = gx0 + c;
+                if (gx < d.hxa) gstore(dst, t_at(d, gx, y), res[lp(c * L + y)]);
+            }
+        }
     }
 }
 
@@ -390,7 +411,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             // 3/2-padded rows (Nax/2 = FL/3), where each of the lane's R0
             // points is known at compile time to be padding, kept or
             // mirrored, except the two runs that straddle FL/3 and 2FL/3.
-            auto load_points = [&]<bool TH>() {
+            // BLK: kx-blocked T (Desc::tlg), else row-major.
+            auto load_points = [&]<bool TH, bool BLK>() {
                 constexpr int M = MI::M;
                 constexpr int H = FL / 3;
                 const int hx = TH ? H : hxa;
@@ -400,7 +422,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     if (!MI::valid(id)) continue;
                     const int y = y0 + MI::seq(id);
                     const int j = MI::bfly(id);
-                    const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
+                    const int yy = y < my ? y : 0;
+                    const uint32_t row = static_cast<uint32_t>(yy * hxa);
                     [&]<int... Q>(std::integer_sequence<int, Q...>) {
                         ([&] {
                             constexpr int q = Q;
@@ -411,8 +434,9 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                                 const int k = j + q * M;
                                 const bool mirror = k > FL - hx;
                                 const int kk = mirror ? FL - k : (k < hx ? k : 0);
-                                v[b][q] = gload(ta, row + kk);
-                                vb[b][q] = gload(tb, row + kk);   // tb = ta when there is no second field
+                                const uint32_t at = BLK ? t_at(d, kk, yy) : row + kk;
+                                v[b][q] = gload(ta, at);
+                                vb[b][q] = gload(tb, at);   // tb = ta when there is no second field
                             }
                         }(), ...);
                     }(std::make_integer_sequence<int, R0>{});
@@ -452,8 +476,13 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     }(std::make_integer_sequence<int, R0>{});
                 }
             };
-            if (3 * hxa == FL) load_points.template operator()<true>();
-            else load_points.template operator()<false>();
+            if (d.tlg < 0) {
+                if (3 * hxa == FL) load_points.template operator()<true, false>();
+                else load_points.template operator()<false, false>();
+            } else {
+                if (3 * hxa == FL) load_points.template operator()<true, true>();
+                else load_points.template operator()<false, true>();
+            }
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
             if constexpr (PAIR) fz_rest_to_regs_pair<T, NT, FL, RF, NBUF>(bf, w, twx, +1, tid);
             else fz_rest_to_regs<T, NT, FL, RF, false, NBUF>(bf, w, twx, +1, tid);
@@ -590,9 +619,10 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const int y = y0 + r;
             const bool mirror = k > L - hxa;
             const int kk = mirror ? L - k : (k < hxa ? k : 0);
-            const uint32_t row = static_cast<uint32_t>((y < my ? y : 0) * hxa);
-            va[i] = gload(ta, row + kk);
-            vb[i] = gload(tb, row + kk);   // tb = ta when there is no second field
+            const int yy = y < my ? y : 0;
+            const uint32_t at = d.tlg < 0 ? static_cast<uint32_t>(yy * hxa + kk) : t_at(d, kk, yy);
+            va[i] = gload(ta, at);
+            vb[i] = gload(tb, at);   // tb = ta when there is no second field
         }
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
@@ -710,16 +740,18 @@ __global__ __launch_bounds__(NT) void k_rows_physical(const InstDev<T>* __restri
             const int y = y0 + r;
             cx<T> z = czero<T>();
             if (y < my) {
-                const uint32_t row = static_cast<uint32_t>(y * hxa);
+                auto at = [&](int kx) {
+                    return d.tlg < 0 ? static_cast<uint32_t>(y * hxa + kx) : t_at(d, kx, y);
+                };
                 if (k < hxa) {
-                    cx<T> a = gload(ta, row + k);
-                    cx<T> b = has_b ? gload(tb, row + k) : czero<T>();
+                    cx<T> a = gload(ta, at(k));
+                    cx<T> b = has_b ? gload(tb, at(k)) : czero<T>();
                     if (k == 0) { a.im = T(0); b.im = T(0); }
                     z = mk(a.re - b.im, a.im + b.re);
                 } else if (k > L - hxa) {
                     const int kk = L - k;
-                    const cx<T> a = gload(ta, row + kk);
-                    const cx<T> b = has_b ? gload(tb, row + kk) : czero<T>();
+                    const cx<T> a = gload(ta, at(kk));
+                    const cx<T> b = has_b ? gload(tb, at(kk)) : czero<T>();
                     z = mk(a.re + b.im, b.re - a.im);
                 }
             }

@@ -849,6 +849,10 @@ struct Tuning {
     // U planes in blocks of 2^ulg kx columns (0: column-major); a block
     // that does not divide Nax/2 falls back to 0
     int ulg = 0;
+    // T planes: -1 row-major, else blocks of 2^tlg kx columns (measured: the
+    // blocks pay where the inverse column pass writes one narrow tile per
+    // workgroup into long rows, e.g. FP32 at 4096^2)
+    int tlg = -1;
 };
 // launch shapes of a shape that was never tuned: default tiles and lanes,
 // the fused transforms with two LDS buffers where they fit (else one), U
@@ -948,6 +952,7 @@ void apply_plans(Instance& h, const Tuning& t) {
         }
     }
     h.desc.ulg = (t.ulg > 0 && t.ulg <= 12 && d.hxa % (1 << t.ulg) == 0) ? t.ulg : 0;
+    h.desc.tlg = (t.tlg >= 0 && t.tlg <= 12 && d.hxa % (1 << t.tlg) == 0) ? t.tlg : -1;
     h.desc.fixed_y = (t.col_transform == 0 || t.col_transform >= 3) ? 1 : 0;
     h.desc.plan_y = make_fft_plan(d.my, t.col_transform == 2 ? 4 : 8);
     h.desc.plan_x = make_fft_plan(d.mx, t.row_transform == 2 ? 4 : 8);
@@ -2307,6 +2312,17 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
         if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
         if (ms[1] + ms[2] < bk) { bk = ms[1] + ms[2]; best.ulg = ulg; best_ms[1] = ms[1]; best_ms[2] = ms[2]; }
     }
+    cur.ulg = best.ulg;
+    // kx blocks of the column-to-row planes: the inverse column pass writes
+    // them, the row pass reads them (their summed time)
+    bk = 1e30;
+    for (int tlg = -1; tlg < 4; ++tlg) {
+        if (tlg >= 0 && d.hxa % (1 << tlg) != 0) break;
+        Tuning t = cur;
+        t.tlg = tlg;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[0] + ms[1] < bk) { bk = ms[0] + ms[1]; best.tlg = tlg; best_ms[0] = ms[0]; best_ms[1] = ms[1]; }
+    }
     store_tuning(h.tune_key, best);
     return SHHD_OK;
 }
@@ -2394,6 +2410,7 @@ shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out
     out->column_transform = best.col_transform;
     out->row_transform = best.row_transform;
     out->plane_block = best.ulg;
+    out->inverse_plane_block = best.tlg + 1;
     for (int k = 0; k < kClasses; ++k) out->ms[k] = ms[k];
     return SHHD_OK;
 }
@@ -2407,10 +2424,11 @@ shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* t) {
     if (t->inverse_column_tile < 0 || t->forward_column_tile < 0 ||
         (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024) ||
         t->column_transform < 0 || t->column_transform >= kTransforms || t->row_transform < 0 ||
-        t->row_transform >= kTransforms || t->plane_block < 0 || t->plane_block > 12)
+        t->row_transform >= kTransforms || t->plane_block < 0 || t->plane_block > 12 || t->inverse_plane_block < 0 ||
+        t->inverse_plane_block > 13)
         return set_err(nullptr, SHHD_ERR_INVALID, "invalid launch shapes");
     store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes, t->column_transform,
-                                     t->row_transform, t->plane_block});
+                                     t->row_transform, t->plane_block, t->inverse_plane_block - 1});
     return SHHD_OK;
 }
 

@@ -123,3 +123,31 @@ def test_plane_blocks_give_identical_results(fp32):
                 assert np.array_equal(out[0][f], ref[0][f]), (ct, pb, f)
             assert np.array_equal(out[1], ref[1]), (ct, pb)
 
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("grid", [(128, 128), (512, 256)])
+def test_inverse_plane_blocks_give_identical_results(fp32, grid):
+    """The kx block of the column-to-row planes only moves where the inverse
+    column pass stores a value and the row pass (and the physical-field row
+    pass) finds it: states, fluxes and physical fields are bit-identical for
+    row-major planes and every block, staged and fused (rows of 192: staged
+    row pass; rows of 768: fused)."""
+    print(f"[case] inverse plane blocks fp32={fp32} grid={grid}", flush=True)
+    cfg = model_h(*grid, fp32=fp32)
+    st = flux_state(cfg)
+
+    def run_phys(shapes):
+        shhd.set_launch_shapes(cfg, *shapes)
+        with shhd.GpuFluidSolver(cfg) as g:
+            g.set_flux_request(density=True, order_parameter=True, momentum=True)
+            g.set_state(st)
+            g.step(3)
+            return g.get_state(), g.get_flux(), g.get_physical()
+
+    for ct, rt in ((0, 0), (4, 4)):
+        ref = run_phys((2, 1, 0, ct, rt, 1, 0))
+        for tb in (1, 2, 3, 4):
+            out = run_phys((2, 1, 0, ct, rt, 1, tb))
+            for k in range(3):
+                assert np.array_equal(out[k], ref[k]), (ct, tb, k)
+

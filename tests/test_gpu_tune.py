@@ -22,6 +22,7 @@ def test_autotune_keeps_results():
     t = shhd.autotune(cfg, steps=2)
     assert t["inverse_column_tile"] in (1, 2, 4, 8) and t["forward_column_tile"] in (1, 2, 4, 8)
     assert t["row_lanes"] in (64, 256, 512, 1024)
+    assert 0 <= t["plane_block"] <= 3 and 0 <= t["inverse_plane_block"] <= 4
     assert all(m > 0 for m in t["ms"])
     with shhd.GpuFluidSolver(cfg) as g:
         g.set_state(st)
