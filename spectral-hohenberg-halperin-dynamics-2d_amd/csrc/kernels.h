@@ -250,6 +250,23 @@ struct RowFields {
             if (alpha == a) v = psi[a][i];
         return v;
     }
+    // f(psi[alpha]) for an order parameter index uniform across the wave:
+    // with up to two fields one scalar branch picks the array (no select per
+    // value); more fields keep the selects (one copy of the code).
+    template <typename F>
+    __device__ __forceinline__ void with_psi(int alpha, F&& f) const {
+        if constexpr (MAXN == 2) {
+            if (alpha == 0) f(psi[0]);
+            else f(psi[1]);
+        } else if constexpr (MAXN <= 1) {
+            f(psi[0]);
+        } else {
+            T p[PPT];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) p[i] = psi_of(alpha, i);
+            f(p);
+        }
+    }
     // Output `it` at every point of the lane: one branch on the output kind
     // per output, with the points in straight-line code inside each case
     // (a branch per point splits the arithmetic into blocks of a few
@@ -258,20 +275,23 @@ struct RowFields {
         switch (it.kind) {
             case OUT_MU: {
                 const T u = T(d.phi4_u[it.alpha]);   // src/model_free_energy_phi4.cc:54
+                with_psi(it.alpha, [&](const T (&p)[PPT]) {
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    const T p = psi_of(it.alpha, i);
-                    out[i] = u * p * p * p;
-                }
+                    for (int i = 0; i < PPT; ++i) out[i] = u * p[i] * p[i] * p[i];
+                });
                 break;
             }
             case OUT_PSIVX:
+                with_psi(it.alpha, [&](const T (&p)[PPT]) {
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) out[i] = psi_of(it.alpha, i) * vx(i);
+                    for (int i = 0; i < PPT; ++i) out[i] = p[i] * vx(i);
+                });
                 break;
             case OUT_PSIVY:
+                with_psi(it.alpha, [&](const T (&p)[PPT]) {
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) out[i] = psi_of(it.alpha, i) * vy(i);
+                    for (int i = 0; i < PPT; ++i) out[i] = p[i] * vy(i);
+                });
                 break;
             case OUT_GRADF: {
 #pragma unroll
