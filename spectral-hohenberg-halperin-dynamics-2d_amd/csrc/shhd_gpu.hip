@@ -1231,6 +1231,18 @@ struct Engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 
+    // Launch classes run on up to kLaneStreams streams (this one and side
+    // streams of default priority): instances of different classes share no
+    // buffers, so a step forks from `stream`, each class runs its stages on
+    // its own stream, and the step joins back before the step counters
+    // advance.  Small grids then fill the GPU beside large ones instead of
+    // waiting for them.  `ls` is the stream the class being issued uses.
+    static constexpr int kLaneStreams = 4;
+    hipStream_t side[kLaneStreams - 1] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kLaneStreams - 1] = {};
+    hipStream_t ls = nullptr;
+
     bool timing = false;
     double time_ms[kClasses] = {0, 0, 0};
     long long launches[kClasses] = {0, 0, 0};
@@ -1239,6 +1251,8 @@ struct Engine {
 
     ~Engine() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (int i = 0; i < kLaneStreams - 1; ++i)
+            if (side[i]) (void)hipStreamSynchronize(side[i]);
         drop_graph();
         for (auto& c : classes) c.release();
         slots.clear();
@@ -1250,8 +1264,29 @@ struct Engine {
             (void)hipEventDestroy(u.second.first);
             (void)hipEventDestroy(u.second.second);
         }
+        for (int i = 0; i < kLaneStreams - 1; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
+
+    // streams the classes of a step are spread over (1: all on `stream`)
+    int lane_streams() const {
+        if (timing || classes.size() < 2) return 1;
+        return std::min<int>(kLaneStreams, static_cast<int>(classes.size()));
+    }
+    shhd_status ensure_side_streams() {
+        if (fork_ev) return SHHD_OK;
+        for (int i = 0; i < kLaneStreams - 1; ++i) {
+            SHHD_HIP(&err, hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
+            SHHD_HIP(&err, hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming));
+        }
+        SHHD_HIP(&err, hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        return SHHD_OK;
+    }
+    hipStream_t class_stream(size_t c, int nls) const { return (c % nls) == 0 ? stream : side[c % nls - 1]; }
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1944,12 +1979,12 @@ struct Engine {
         if (t) {
             a = take_event();
             b = take_event();
-            SHHD_HIP(&err, hipEventRecord(a, stream));
+            SHHD_HIP(&err, hipEventRecord(a, ls));
         }
         launch();
         SHHD_HIP(&err, hipGetLastError());
         if (t) {
-            SHHD_HIP(&err, hipEventRecord(b, stream));
+            SHHD_HIP(&err, hipEventRecord(b, ls));
             ev_used.push_back({cls, {a, b}});
         }
         return SHHD_OK;
@@ -1965,20 +2000,20 @@ struct Engine {
         if (lc.nblk[0] > 0) {
             K1Fn<T> k1 = pick_k1<T>(K.cNT, K.cV, K.cfz, K.cnb);
             st = timed(0, capturing, [&] {
-                hipLaunchKernelGGL(k1, dim3(lc.nblk[0]), dim3(K.cNT), lc.lds[0], stream, inst, ptrs, lc.map[0]);
+                hipLaunchKernelGGL(k1, dim3(lc.nblk[0]), dim3(K.cNT), lc.lds[0], ls, inst, ptrs, lc.map[0]);
             });
             if (st != SHHD_OK) return st;
         }
         if (lc.nblk[1] > 0) {
             K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay);
             st = timed(1, capturing, [&] {
-                hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], stream, inst, lc.map[1]);
+                hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], ls, inst, lc.map[1]);
             });
             if (st != SHHD_OK) return st;
         }
         K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb);
         return timed(2, capturing, [&] {
-            hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.c3NT), lc.lds[2], stream, inst, ptrs, sp, lc.map[2]);
+            hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.c3NT), lc.lds[2], ls, inst, ptrs, sp, lc.map[2]);
         });
     }
 
@@ -1990,19 +2025,31 @@ struct Engine {
         // time_integrator_srk3_*.h: (w_base, w_current) = (0,1), (3/4,1/4), (1/3,2/3)
         static constexpr double WB[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
         static constexpr double WC[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-        for (auto& lc : classes) {
+        // fork: class c runs on stream c mod nls
+        const int nls = lane_streams();
+        if (nls > 1) {
+            const shhd_status st = ensure_side_streams();
+            if (st != SHHD_OK) return st;
+            SHHD_HIP(&err, hipEventRecord(fork_ev, stream));
+            for (int i = 0; i < nls - 1; ++i) SHHD_HIP(&err, hipStreamWaitEvent(side[i], fork_ev, 0));
+        }
+        for (size_t c = 0; c < classes.size(); ++c) {
+            LaunchClass& lc = classes[c];
             if (!lc.nstatic) continue;
+            ls = class_stream(c, nls);
             if (lc.key.fp32)
-                hipLaunchKernelGGL(k_enforce_static<float>, dim3(lc.nstatic), dim3(256), 0, stream,
+                hipLaunchKernelGGL(k_enforce_static<float>, dim3(lc.nstatic), dim3(256), 0, ls,
                                    static_cast<const StaticEntry<float>*>(lc.d_static));
             else
-                hipLaunchKernelGGL(k_enforce_static<double>, dim3(lc.nstatic), dim3(256), 0, stream,
+                hipLaunchKernelGGL(k_enforce_static<double>, dim3(lc.nstatic), dim3(256), 0, ls,
                                    static_cast<const StaticEntry<double>*>(lc.d_static));
             SHHD_HIP(&err, hipGetLastError());
         }
         for (int s = 0; s < 3; ++s) {
-            for (auto& lc : classes) {
+            for (size_t c = 0; c < classes.size(); ++c) {
+                LaunchClass& lc = classes[c];
                 if (s >= lc.key.stages) continue;
+                ls = class_stream(c, nls);
                 // stage weights of the step flux: 1/6, 1/6, 2/3 (SRK3) or 1 (Euler)
                 static constexpr double FW[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
                 StageParams sp{};
@@ -2030,11 +2077,21 @@ struct Engine {
                 if (st != SHHD_OK) return st;
             }
         }
-        for (auto& lc : classes) {
+        for (size_t c = 0; c < classes.size(); ++c) {
+            LaunchClass& lc = classes[c];
             if (!lc.d_meas) continue;
+            ls = class_stream(c, nls);
             shhd_status st = lc.key.fp32 ? launch_measures<float>(lc, k) : launch_measures<double>(lc, k);
             if (st != SHHD_OK) return st;
         }
+        // join: the step counters advance once every class has finished the step
+        if (nls > 1) {
+            for (int i = 0; i < nls - 1; ++i) {
+                SHHD_HIP(&err, hipEventRecord(join_ev[i], side[i]));
+                SHHD_HIP(&err, hipStreamWaitEvent(stream, join_ev[i], 0));
+            }
+        }
+        ls = stream;
         hipLaunchKernelGGL(k_advance, dim3((kMaxSlots + 255) / 256), dim3(256), 0, stream, d_steps, d_flags, d_active,
                            kMaxSlots);
         SHHD_HIP(&err, hipGetLastError());
@@ -2048,9 +2105,9 @@ struct Engine {
         const MeasDev<T>* meas = static_cast<const MeasDev<T>*>(lc.d_meas);
         const StagePtrs<T>* last =
             static_cast<const StagePtrs<T>*>(lc.d_ptrs) + static_cast<size_t>(k * lc.key.stages + lc.key.stages - 1) * n;
-        if (lc.has_probe) hipLaunchKernelGGL(k_probe<T>, dim3(n), dim3(256), 0, stream, meas, last);
+        if (lc.has_probe) hipLaunchKernelGGL(k_probe<T>, dim3(n), dim3(256), 0, ls, meas, last);
         if (lc.has_sk)
-            hipLaunchKernelGGL(k_structure_factor<T>, dim3(lc.sk_blocks, n), dim3(256), 0, stream, meas, last);
+            hipLaunchKernelGGL(k_structure_factor<T>, dim3(lc.sk_blocks, n), dim3(256), 0, ls, meas, last);
         SHHD_HIP(&err, hipGetLastError());
         return SHHD_OK;
     }
@@ -2078,6 +2135,10 @@ struct Engine {
         }
         if (dirty) {
             shhd_status st = rebuild();
+            if (st != SHHD_OK) return st;
+        }
+        if (lane_streams() > 1) {   // created before any graph capture begins
+            const shhd_status st = ensure_side_streams();
             if (st != SHHD_OK) return st;
         }
         for (auto& sp : slots) {
