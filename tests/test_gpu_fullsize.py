@@ -228,3 +228,23 @@ def test_4096_fused_against_runtime_plan(name, precision, shapes, tol):
     errs = [rel_l2(got[f], ref[f]) for f in range(cfg.num_fields) if np.linalg.norm(ref[f]) > 0]
     print(f"    per-field rel L2 {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) <= tol
+
+
+@pytest.mark.parametrize("name,precision,shapes", [("model_h", 64, (1, 1, 1024, 3, 3, 2)),
+                                                   ("model_h", 32, (1, 1, 1024, 4, 4, 2)),
+                                                   ("incompressible", 32, (1, 1, 1024, 4, 3, 2))])
+def test_4096_inverse_plane_blocks_bitwise(name, precision, shapes):
+    """The 6144-point fused builds with kx-blocked column-to-row planes (the
+    layout the autotuner picks for the FP32 form at 4096^2) give the states
+    of row-major planes bit for bit: only where K1 stores and K2 finds a value
+    moves."""
+    banner(f"{name} 4096^2 inverse plane blocks fp{precision}")
+    if name == "model_h":
+        cfg, ics = model_h(4096, precision, noise=True), model_h_4096_ics()
+    else:
+        cfg, ics = incompressible_4096(precision), incompressible_4096_ics()
+    _, ref, _ = run_gpu(cfg, shapes + (0,), ics=ics)
+    for tb in (1, 3):
+        _, got, info = run_gpu(cfg, shapes + (tb,), ics=ics)
+        assert info.row_fused > 0 and info.inverse_column_fused > 0
+        assert np.array_equal(got, ref), tb
