@@ -321,10 +321,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
-    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT[,CT,RT,PB[,TB]]",
+    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT[,CT,RT,PB[,TB[,FW[,RW]]]]",
                     help="use these launch shapes (inverse tile, forward tile, row lanes, column and row "
-                         "transform choice, log2 row-to-column plane block, column-to-row plane block "
-                         "as in shhd_tuning) instead of tuning")
+                         "transform choice, log2 row-to-column plane block, column-to-row plane block, "
+                         "forward-column and row wave groups, as in shhd_tuning) instead of tuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
@@ -339,11 +339,12 @@ def main():
     cfg, desc = workload(args.workload, args.precision)
     # launch shapes for this problem shape, measured before the timed region
     if args.launch_shapes:
-        vals = [int(v) for v in args.launch_shapes.split(",")] + [0, 0, 0, 0]
-        c1, c3, rnt, ct, rt, pb, tb = vals[:7]
-        shhd.set_launch_shapes(cfg, c1, c3, rnt, ct, rt, pb, tb)
+        vals = [int(v) for v in args.launch_shapes.split(",")] + [0, 0, 0, 0, 0, 0]
+        c1, c3, rnt, ct, rt, pb, tb, fw, rw = vals[:9]
+        shhd.set_launch_shapes(cfg, c1, c3, rnt, ct, rt, pb, tb, fw, rw)
         tuning = {"inverse_column_tile": c1, "forward_column_tile": c3, "row_lanes": rnt, "column_transform": ct,
-                  "row_transform": rt, "plane_block": pb, "inverse_plane_block": tb, "given": True}
+                  "row_transform": rt, "plane_block": pb, "inverse_plane_block": tb, "forward_column_waves": fw,
+                  "row_waves": rw, "given": True}
     else:
         tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
     ics = device_initial_conditions(args.workload)

@@ -311,6 +311,12 @@ typedef struct shhd_tuning {
                                   k > 0 blocks of 2^(k-1) columns laid out as
                                   the row-to-column planes' (a block that
                                   does not divide Nax/2 uses row-major)      */
+    int forward_column_waves;  /* forward-column workgroups of one-wave fused
+                                  tiles: 0 four wave groups splitting the
+                                  products where a launch has few workgroups
+                                  (the default), 1 never, 4 wherever built */
+    int row_waves;             /* row-pass workgroups of one-wave staged row
+                                  tiles: the same choices                    */
     double ms[3];              /* mean launch time of each kernel class (as
                                   shhd_gpu_get_kernel_timing) with that shape */
 } shhd_tuning;

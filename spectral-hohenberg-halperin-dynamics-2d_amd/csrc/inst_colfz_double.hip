@@ -4,4 +4,6 @@
 namespace shhd {
 #define SHHD_B(nt, v, len) SHHD_I_COLFZ(template, double, nt, v, len)
 SHHD_FOR_COL_FZ(SHHD_B)
+#define SHHD_W(nt, v, len) SHHD_I_K3W(template, double, nt, v, len)
+SHHD_FOR_COL_FZ_WS(SHHD_W)
 }  // namespace shhd

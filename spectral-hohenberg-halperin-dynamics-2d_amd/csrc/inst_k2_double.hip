@@ -4,4 +4,6 @@
 namespace shhd {
 #define SHHD_B(nt, v, fl) SHHD_I_ROWFL(template, double, nt, v, fl)
 SHHD_FOR_ROW_FL(SHHD_B)
+#define SHHD_W(nt, v) SHHD_FOR_MAXN(SHHD_I_K2W, template, double, nt, v)
+SHHD_FOR_ROW_WS(SHHD_W)
 }  // namespace shhd

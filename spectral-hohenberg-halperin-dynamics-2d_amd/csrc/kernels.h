@@ -340,10 +340,11 @@ struct RowFields {
 // that separate 3 from 2 resident waves per SIMD.
 template <typename T, int NT, int PPT, int FL>
 __device__ __forceinline__ void row_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
-                                        const cx<T>* __restrict__ tw, int sign) {
-    if constexpr (FL > 0) lds_fft_ct<T, NT, PPT, FL>(buf, nseq, tw, sign);
-    else if constexpr (sizeof(T) == 4) lds_fft_any<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
-    else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign);
+                                        const cx<T>* __restrict__ tw, int sign,
+                                        int tid = static_cast<int>(threadIdx.x)) {
+    if constexpr (FL > 0) lds_fft_ct<T, NT, PPT, FL>(buf, nseq, tw, sign, tid);
+    else if constexpr (sizeof(T) == 4) lds_fft_any<T, NT, PPT>(buf, L, nseq, plan, tw, sign, tid);
+    else lds_fft<T, NT, PPT>(buf, L, nseq, plan, tw, sign, tid);
 }
 
 // Inverse-transformed fields in a fixed order, known at compile time:
@@ -358,6 +359,14 @@ constexpr int layout_role(int layout, int maxn, int p) {
     return p == maxn ? ROLE_JX : ROLE_JY;
 }
 
+// WS > 1 (the staged any-length build of short rows, where a launch has few
+// workgroups): the workgroup's WS wave groups of NT / WS lanes hold the same
+// R rows; group g inverse-transforms the field pairs g, g + WS, ... into its
+// own LDS buffer and parks the physical fields in a shared LDS plane set,
+// then every group reads all fields at its points and forward-transforms the
+// product pairs g, g + WS, ...  Every group runs the same number of
+// transforms (zeros where it has no pair left), so the barriers line up.
+//
 // NBUF > 0 (FL > 0 and PPT = 6 only): the inverse transform runs with the
 // radix-3 pass last and leaves each lane the six points j, j + FL/3, j + 2FL/3
 // of its two radix-3 butterflies; the products at those points feed the
@@ -365,7 +374,7 @@ constexpr int layout_role(int layout, int maxn, int p) {
 // inverse's first pass loads T straight into its butterflies, and the passes
 // alternate between NBUF LDS buffers.  NBUF = 0: every transform staged
 // through one LDS buffer.
-template <typename T, int NT, int PPT, int MAXN, int FL, int NBUF, int LAYOUT>
+template <typename T, int NT, int PPT, int MAXN, int FL, int NBUF, int LAYOUT, int WS>
 __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ insts, BlockMap map) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
@@ -393,6 +402,131 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
 
     RowFields<T, PPT, MAXN> F;
     F.clear();
+
+    if constexpr (WS > 1) {
+        static_assert(FL == 0 && NBUF == 0 && LAYOUT == 0 && NT % WS == 0, "wave groups split the staged row pass");
+        constexpr int NTF = NT / WS;
+        const int grp = tid / NTF;
+        const int tidf = tid - grp * NTF;
+        const int lsz = lp_size(npts);   // one group's transform buffer (R L <= NTF PPT, host-checked)
+        cx<T>* const gbuf = lds + static_cast<size_t>(grp) * lsz;
+        T* const phys = reinterpret_cast<T*>(lds + static_cast<size_t>(WS) * lsz);   // n_in planes of R L
+        // ---- inverse x transforms: field pair pr by group pr mod WS --------
+        const int npairs = (d.n_in + 1) / 2;
+        const int rounds_in = (npairs + WS - 1) / WS;
+        for (int it = 0; it < rounds_in; ++it) {
+            const int pr = it * WS + grp;
+            const bool mine = pr < npairs;
+            const int s = mine ? 2 * pr : 0;
+            const bool has_b = mine && s + 1 < d.n_in;
+            const cx<T>* __restrict__ ta = tbuf + static_cast<size_t>(s) * plane;
+            const cx<T>* __restrict__ tb = tbuf + static_cast<size_t>(has_b ? s + 1 : s) * plane;
+            cx<T> va[PPT], vb[PPT];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int idx = tidf + NTF * i;
+                const int r = idx / L;
+                const int k = idx - r * L;
+                const int y = y0 + r;
+                const bool mirror = k > L - hxa;
+                const int kk = mirror ? L - k : (k < hxa ? k : 0);
+                const int yy = y < my ? y : 0;
+                const uint32_t at = d.tlg < 0 ? static_cast<uint32_t>(yy * hxa + kk) : t_at(d, kk, yy);
+                va[i] = mine ? gload(ta, at) : czero<T>();
+                vb[i] = mine ? gload(tb, at) : czero<T>();
+            }
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int idx = tidf + NTF * i;
+                if (idx >= npts) continue;
+                const int r = idx / L;
+                const int k = idx - r * L;
+                const bool mirror = k > L - hxa;
+                const bool live = mine && y0 + r < my && (k < hxa || mirror);
+                cx<T> a = va[i];
+                cx<T> b = has_b ? vb[i] : czero<T>();
+                a.im = k == 0 ? T(0) : (mirror ? -a.im : a.im);
+                b.im = k == 0 ? T(0) : (mirror ? -b.im : b.im);
+                gbuf[lp(idx)] = live ? mk(a.re - b.im, a.im + b.re) : czero<T>();   // A + iB
+            }
+            __syncthreads();
+            row_fft<T, NTF, PPT, FL>(gbuf, L, R, d.plan_x, twx, +1, tidf);
+            if (mine) {
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const int idx = tidf + NTF * i;
+                    if (idx >= npts) continue;
+                    const cx<T> z = gbuf[lp(idx)];
+                    phys[s * npts + idx] = z.re;
+                    if (has_b) phys[(s + 1) * npts + idx] = z.im;
+                }
+            }
+            __syncthreads();
+        }
+        // every group: all fields at its points
+        for (int f = 0; f < d.n_in; ++f) {
+            const int role = d.in_role[f];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int idx = tidf + NTF * i;
+                F.store(role, i, idx < npts ? phys[f * npts + idx] : T(0));
+            }
+        }
+        // ---- velocity (src/fcalculator_workspace.cc:122-133) ---------------
+        if (d.make_velocity) {
+            const T r0 = T(d.rho0);
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int idx = tidf + NTF * i;
+                if (idx < npts && F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + idx / L < my)
+                    atomicOr(flag, 1);
+            }
+        }
+        // ---- products and forward x transforms: product pair q by group q mod WS
+        const int nopairs = (d.n_out + 1) / 2;
+        const int rounds_out = (nopairs + WS - 1) / WS;
+        for (int it = 0; it < rounds_out; ++it) {
+            const int q = it * WS + grp;
+            const bool mine = q < nopairs;
+            const int o = mine ? 2 * q : 0;
+            const bool has_b = mine && o + 1 < d.n_out;
+            const OutItem ia = d.out[o];
+            const OutItem ib = has_b ? d.out[o + 1] : d.out[o];
+            F.fence();
+            T va[PPT], vb[PPT];
+            F.eval_all(d, ia, va);
+            if (has_b) F.eval_all(d, ib, vb);
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int idx = tidf + NTF * i;
+                if (idx < npts) gbuf[lp(idx)] = mine ? mk(va[i], has_b ? vb[i] : T(0)) : czero<T>();
+            }
+            __syncthreads();
+            row_fft<T, NTF, PPT, FL>(gbuf, L, R, d.plan_x, twx, -1, tidf);
+            if (mine) {
+                cx<T>* __restrict__ ua = ubuf + static_cast<size_t>(o) * plane;
+                cx<T>* __restrict__ ub = ubuf + static_cast<size_t>(has_b ? o + 1 : o) * plane;
+                const int nk = R * hxa;
+                const int umask = (1 << d.ulg) - 1;
+                for (int idx = tidf; idx < nk; idx += NTF) {
+                    const int kb = idx / (R << d.ulg);
+                    const int rem = idx - kb * (R << d.ulg);
+                    const int r = rem >> d.ulg;
+                    const int k = (kb << d.ulg) + (rem & umask);
+                    const int y = y0 + r;
+                    if (y < my) {
+                        const cx<T> zk = gbuf[lp(r * L + k)];
+                        const cx<T> zm = gbuf[lp(r * L + (k == 0 ? 0 : L - k))];
+                        const uint32_t at = ((static_cast<uint32_t>(kb) * my + y) << d.ulg) + (k & umask);   // kx-blocked U
+                        gstore(ua, at, mk(T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)));
+                        if (has_b) gstore(ub, at, mk(T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
 
     if constexpr (NBUF > 0) {
         static_assert(FL > 0 && PPT % 3 == 0, "fused row transforms need a fixed length and radix-3 butterflies per lane");
@@ -849,7 +983,17 @@ __device__ __forceinline__ void k0_write(const cx<double>* base_k0, const cx<dou
 // (output j + 2FZL/3), the only kept outputs, so every lane accumulates two
 // modes per butterfly and no LDS read-back is needed.  The passes alternate
 // between NBUF LDS buffers.
-template <typename T, int NT, int VPT, int ACC, int FZL, int NBUF>
+//
+// WS > 1 (small columns, few workgroups): the workgroup's WS wave groups of
+// NT / WS lanes each transform their own products of the same column tile
+// (product i by group i mod WS) with their own LDS buffers, park their
+// accumulated terms side by side, and the stage update, spread over all NT
+// lanes, adds the WS partial sums of each mode.  Every group runs the same
+// number of transforms (a group out of products transforms zeros and keeps
+// nothing), so the barriers inside the transforms line up.  While the step
+// flux is recorded, group 0 takes every product (the flux slots of a mode
+// are then added to in one order, by one lane).
+template <typename T, int NT, int VPT, int ACC, int FZL, int NBUF, int WS>
 __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __restrict__ insts,
                                                         const StagePtrs<T>* __restrict__ ptrs, StageParams sp,
                                                         BlockMap map) {
@@ -971,34 +1115,50 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
         }
     };
 
+    // WS partial sums per mode, side by side (WS = 1: the sums themselves)
     cx<T>* acc_x = lds;
-    cx<T>* acc_y = lds + nmodes;
+    cx<T>* acc_y = lds + WS * nmodes;
+    static_assert(WS == 1 || FZL > 0, "wave groups split the fused column transforms only");
     if constexpr (FZL > 0) {
-        constexpr int CF = NT * VPT / FZL;
+        constexpr int NTF = NT / WS;                // lanes of one transform group
+        constexpr int CF = NTF * VPT / FZL;
         constexpr int M3 = FZL / 3;
         constexpr int R0 = fz_radix(FZL, false, 0);
         constexpr int NP = fz_npass(FZL, false);
         constexpr int B3 = VPT / 3;                 // radix-3 butterflies per lane
         static_assert(fz_radix(FZL, false, NP - 1) == 3 && ACC == 2 * B3, "radix-3 last pass, two modes per butterfly");
-        static_assert(FzMap<NT, FZL, CF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
-        using M0 = FzMap<NT, FZL, CF, R0>;
+        static_assert(FzMap<NTF, FZL, CF, 3>::NB == B3 * NTF, "whole radix-3 butterflies per lane");
+        static_assert(NT % WS == 0 && NTF % 64 == 0, "whole waves per group");
+        using M0 = FzMap<NTF, FZL, CF, R0>;
+        const int grp = WS > 1 ? tid / NTF : 0;     // transform group of this lane
+        const int tidf = tid - grp * NTF;           // lane within the group
         // The last radix-8 and the radix-3 pass exchange across lanes (FP64
         // below 1024 lanes, as the row pass); the kept modes then follow the
         // FzGrp map
-        constexpr bool PAIR = sizeof(T) == 8 && NT < 1024 && VPT == 6 && fz_pair_ok<FZL, false>();
-        auto seq_of = [&](int b, int lane_) { return PAIR ? FzGrp<FZL>(lane_).s : FzMap<NT, FZL, CF, 3>::seq(lane_ + NT * b); };
+        constexpr bool PAIR = sizeof(T) == 8 && NTF < 1024 && VPT == 6 && fz_pair_ok<FZL, false>();
+        auto seq_of = [&](int b, int lane_) { return PAIR ? FzGrp<FZL>(lane_).s : FzMap<NTF, FZL, CF, 3>::seq(lane_ + NTF * b); };
         auto bfly_of = [&](int b, int lane_) {
-            return PAIR ? FzGrp<FZL>(lane_).j3(b) : FzMap<NT, FZL, CF, 3>::bfly(lane_ + NT * b);
+            return PAIR ? FzGrp<FZL>(lane_).j3(b) : FzMap<NTF, FZL, CF, 3>::bfly(lane_ + NTF * b);
         };
-        FzBufs<T, NBUF> bf{lds, lds + lp_size(CF * FZL), 0};
-        // the first butterflies of contribution ci, straight from U
+        cx<T>* const gbuf = lds + static_cast<size_t>(grp) * NBUF * lp_size(CF * FZL);
+        FzBufs<T, NBUF> bf{gbuf, gbuf + lp_size(CF * FZL), 0};
+        // the first butterflies of contribution ci, straight from U (zeros for
+        // ci = none: a group with no product left this round)
+        const int ncon = g.ncontrib;
         auto load_contrib = [&](int ci, cx<T> (&v)[M0::B][R0]) {
+            if (ci >= ncon) {
+#pragma unroll
+                for (int b = 0; b < M0::B; ++b)
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) v[b][q] = czero<T>();
+                return;
+            }
             const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(g.contrib[ci]) * plane;
-            int lane0 = tid;
+            int lane0 = tidf;
             asm volatile("" : "+v"(lane0));
 #pragma unroll
             for (int b = 0; b < M0::B; ++b) {
-                const int id = lane0 + NT * b;
+                const int id = lane0 + NTF * b;
                 if (!M0::valid(id)) continue;
                 const int s = M0::seq(id);
                 const int j = M0::bfly(id);
@@ -1009,20 +1169,26 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     v[b][q] = gx < hxa ? gload(src, col + (static_cast<uint32_t>(j + q * M0::M) << d.ulg)) : czero<T>();
             }
         };
+        // round it of group grp transforms product ci_of(it) (ncon: none)
+        const bool split = WS > 1 && !recording;
+        const int rounds = split ? (ncon + WS - 1) / WS : ncon;
+        auto ci_of = [&](int it) { return split ? it * WS + grp : (grp == 0 ? it : ncon); };
         cx<T> v[M0::B][R0];
-        if (g.ncontrib > 0) load_contrib(0, v);
-        for (int ci = 0; ci < g.ncontrib; ++ci) {
-            const ContribCoef& cf = g.coef[ci];
-            fz_first<T, NT, FZL, CF, false, NBUF>(bf, v, -1, tid);
+        if (rounds > 0) load_contrib(ci_of(0), v);
+        for (int it = 0; it < rounds; ++it) {
+            const int ci = ci_of(it);
+            const ContribCoef& cf = g.coef[ci < ncon ? ci : 0];
+            fz_first<T, NTF, FZL, CF, false, NBUF>(bf, v, -1, tidf);
             // the next contribution's loads fly while this one's passes run
             // (the barriers wait for LDS operations only)
-            if (ci + 1 < g.ncontrib) load_contrib(ci + 1, v);
+            if (it + 1 < rounds) load_contrib(ci_of(it + 1), v);
             cx<T> w[B3][3];
-            if constexpr (PAIR) fz_rest_to_regs_pair<T, NT, FZL, CF, NBUF>(bf, w, twy, -1, tid);
-            else fz_rest_to_regs<T, NT, FZL, CF, false, NBUF>(bf, w, twy, -1, tid);
+            if constexpr (PAIR) fz_rest_to_regs_pair<T, NTF, FZL, CF, NBUF>(bf, w, twy, -1, tidf);
+            else fz_rest_to_regs<T, NTF, FZL, CF, false, NBUF>(bf, w, twy, -1, tidf);
+            if (ci >= ncon) continue;
             // an opaque copy of the lane index: the mode coordinates below are
             // recomputed per contribution instead of held live across the transform
-            int lane = tid;
+            int lane = tidf;
             asm volatile("" : "+v"(lane));
 #pragma unroll
             for (int b = 0; b < B3; ++b) {
@@ -1036,20 +1202,22 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
 
         }
         // park the accumulated terms at their modes once every lane is done
-        // with the transform buffers
+        // with the transform buffers (group grp's sums at grp * nmodes)
         __syncthreads();
+        cx<T>* const px = acc_x + grp * nmodes;
+        cx<T>* const py = acc_y + grp * nmodes;
 #pragma unroll
         for (int b = 0; b < B3; ++b) {
-            const int s = seq_of(b, tid);
-            const int j = bfly_of(b, tid);
+            const int s = seq_of(b, tidf);
+            const int j = bfly_of(b, tidf);
             if (gx0 + s >= hxa) continue;
             if (j < half) {
-                acc_x[s * nay + j] = acc0[2 * b];
-                acc_y[s * nay + j] = acc1[2 * b];
+                px[s * nay + j] = acc0[2 * b];
+                py[s * nay + j] = acc1[2 * b];
             }
             if (M3 - j < half) {
-                acc_x[s * nay + j - M3 + nay] = acc0[2 * b + 1];
-                acc_y[s * nay + j - M3 + nay] = acc1[2 * b + 1];
+                px[s * nay + j - M3 + nay] = acc0[2 * b + 1];
+                py[s * nay + j - M3 + nay] = acc1[2 * b + 1];
             }
         }
         __syncthreads();
@@ -1095,6 +1263,13 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     }
 
     // ---- spectral terms, noise, stage update --------------------------------
+    // the product terms of mode m: the sum of the WS groups' parked sums
+    auto acc_sum = [&](const cx<T>* acc, int m) {
+        cx<T> r = acc[m];
+#pragma unroll
+        for (int w = 1; w < WS; ++w) r = r + acc[w * nmodes + m];
+        return r;
+    };
     const int a = g.alpha;
     const bool compressible = d.mode == SHHD_MODE_COMPRESSIBLE;
     const int fx_field = d.N + 1, fy_field = d.N + 2;
@@ -1146,7 +1321,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 const int f = 1 + a;
                 const cx<T> cur = gload(P.cur[f], at);
                 const cx<T> bas = use_base ? gload(P.base[f], at) : czero<T>();
-                cx<T> det = acc_x[m];
+                cx<T> det = acc_sum(acc_x, m);
                 if (d.psi_det_on[a]) {
                     if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
                         const T mob = T(d.mobility[a]);
@@ -1208,7 +1383,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                 const cx<T> bx = use_base ? gload(P.base[fx_field], at) : czero<T>();
                 const cx<T> by = use_base ? gload(P.base[fy_field], at) : czero<T>();
                 const cx<T> brho = (use_base && compressible) ? gload(P.base[0], at) : czero<T>();
-                cx<T> dx = acc_x[m], dy = acc_y[m];
+                cx<T> dx = acc_sum(acc_x, m), dy = acc_sum(acc_y, m);
                 if (d.mom_det_on) {
                     if (pressure) {   // src/fcalculator.cc:276-300
                         const cx<T> p = T(d.pressure_coefficient) * rho;

@@ -138,9 +138,11 @@ __device__ __forceinline__ void mul_twiddles(cx<T> (&v)[R], const Twiddles<T, R>
     }
 }
 
+// tid: the lane within the NT lanes that run the transform (a workgroup may
+// hold several such groups, each with its own buffer)
 template <typename T, int NT, int VPT, int R>
 __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftPass& ps, const cx<T>* __restrict__ tw,
-                                         int sign) {
+                                         int sign, int tid) {
     constexpr int B = (VPT + R - 1) / R;
     const int m = ps.m;
     const int nb = nseq * m;
@@ -149,7 +151,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
     int jj[B], base[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int id = static_cast<int>(threadIdx.x) + NT * b;
+        const int id = tid + NT * b;
         jj[b] = 0;
         base[b] = 0;
         if (id < nb) {
@@ -168,7 +170,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int id = static_cast<int>(threadIdx.x) + NT * b;
+        const int id = tid + NT * b;
         if (id < nb) {
             const int j = jj[b];
             const int k = j & ns_mask;
@@ -186,13 +188,14 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int L, int nseq, const FftP
 // Requires nseq * L <= NT * VPT and the plan's radices to multiply to L.
 template <typename T, int NT, int VPT>
 __device__ __forceinline__ void lds_fft(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
-                                        const cx<T>* __restrict__ tw, int sign) {
+                                        const cx<T>* __restrict__ tw, int sign,
+                                        int tid = static_cast<int>(threadIdx.x)) {
     for (int p = 0; p < plan.npass; ++p) {
         const FftPass& ps = plan.pass[p];
-        if (ps.radix == 8) fft_pass<T, NT, VPT, 8>(buf, L, nseq, ps, tw, sign);
-        else if (ps.radix == 4) fft_pass<T, NT, VPT, 4>(buf, L, nseq, ps, tw, sign);
-        else if (ps.radix == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ps, tw, sign);
-        else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ps, tw, sign);
+        if (ps.radix == 8) fft_pass<T, NT, VPT, 8>(buf, L, nseq, ps, tw, sign, tid);
+        else if (ps.radix == 4) fft_pass<T, NT, VPT, 4>(buf, L, nseq, ps, tw, sign, tid);
+        else if (ps.radix == 3) fft_pass<T, NT, VPT, 3>(buf, L, nseq, ps, tw, sign, tid);
+        else fft_pass<T, NT, VPT, 2>(buf, L, nseq, ps, tw, sign, tid);
     }
 }
 
@@ -295,8 +298,8 @@ __device__ __forceinline__ void lds_fft_ct_passes(cx<T>* buf, int nseq, const cx
 }
 
 template <typename T, int NT, int VPT, int L>
-__device__ __forceinline__ void lds_fft_ct(cx<T>* buf, int nseq, const cx<T>* __restrict__ tw, int sign) {
-    int tid = static_cast<int>(threadIdx.x);
+__device__ __forceinline__ void lds_fft_ct(cx<T>* buf, int nseq, const cx<T>* __restrict__ tw, int sign,
+                                           int tid = static_cast<int>(threadIdx.x)) {
     // an opaque copy of the lane index: the constant-folded addresses of this
     // call are then formed here, not hoisted out of the caller's loops and
     // held live across them
@@ -308,19 +311,20 @@ __device__ __forceinline__ void lds_fft_ct(cx<T>* buf, int nseq, const cx<T>* __
 // launch shape, else the runtime plan.
 template <typename T, int NT, int VPT>
 __device__ __forceinline__ void lds_fft_any(cx<T>* buf, int L, int nseq, const FftPlanDev& plan,
-                                            const cx<T>* __restrict__ tw, int sign) {
+                                            const cx<T>* __restrict__ tw, int sign,
+                                            int tid = static_cast<int>(threadIdx.x)) {
     constexpr int CAP = NT * VPT;
 #define SHHD_FFT_CT(LEN)                                                  \
     if constexpr (VPT == 6 && LEN <= CAP && 8 * LEN >= CAP) {             \
         if (L == LEN) {                                                   \
-            lds_fft_ct<T, NT, VPT, LEN>(buf, nseq, tw, sign);             \
+            lds_fft_ct<T, NT, VPT, LEN>(buf, nseq, tw, sign, tid);        \
             return;                                                       \
         }                                                                 \
     }
     SHHD_FFT_CT(96) SHHD_FFT_CT(192) SHHD_FFT_CT(384) SHHD_FFT_CT(768)
     SHHD_FFT_CT(1536) SHHD_FFT_CT(3072) SHHD_FFT_CT(6144)
 #undef SHHD_FFT_CT
-    lds_fft<T, NT, VPT>(buf, L, nseq, plan, tw, sign);
+    lds_fft<T, NT, VPT>(buf, L, nseq, plan, tw, sign, tid);
 }
 
 }  // namespace shhd

@@ -78,6 +78,7 @@ struct ColPlan {
     int nbuf = 1;      // LDS buffers of the fused build
     size_t lds = 0;    // K1: padded C*My values
     size_t lds3 = 0;   // K3: max(padded C*My, 2*C*Nay) values (FFT column, then both accumulators)
+    int ws = 1;        // K3: wave groups of 64 lanes splitting the products of a tile (fused builds)
 };
 struct RowPlan {
     int R = 1, NT = 256, V = 6, nblocks = 1, maxn = 0;
@@ -85,6 +86,7 @@ struct RowPlan {
     int nbuf = 0;      // fixed-length build: 0 staged, 1 or 2 fused with that many LDS buffers
     int layout = 0;    // fused build: fixed order of the inverse-transformed fields (0: any)
     size_t lds = 0;
+    int ws = 1;        // wave groups of 64 lanes splitting the transforms of a tile (staged builds)
 };
 
 // device transform lengths: 2^a or 3 * 2^a (every Stockham sub-length ns is
@@ -180,6 +182,22 @@ using K2Fn = void (*)(const InstDev<T>*, BlockMap);
 template <typename T>
 using K3Fn = void (*)(const InstDev<T>*, const StagePtrs<T>*, StageParams, BlockMap);
 
+// forward-column builds with wave groups exist for (NT, V, L)
+bool col_ws_built(int NT, int V, int L) {
+#define SHHD_CWS(nt, v, len) if (NT == nt && V == v && L == len) return true;
+    SHHD_FOR_COL_FZ_WS(SHHD_CWS)
+#undef SHHD_CWS
+    return false;
+}
+
+// row builds with wave groups exist for (NT, V)
+bool row_ws_built(int NT, int V) {
+#define SHHD_RWS(nt, v) if (NT == nt && V == v) return true;
+    SHHD_FOR_ROW_WS(SHHD_RWS)
+#undef SHHD_RWS
+    return false;
+}
+
 // fused column builds exist for (NT, V, L)
 bool col_fused_built(int NT, int V, int L) {
 #define SHHD_CFZ(nt, v, len) if (NT == nt && V == v && L == len) return true;
@@ -205,19 +223,29 @@ K1Fn<T> pick_k1(int NT, int V, int fz, int nbuf) {
 }
 
 template <typename T>
-K3Fn<T> pick_k3(int NT, int V, int acc, int fz, int nbuf) {
+K3Fn<T> pick_k3(int NT, int V, int acc, int fz, int nbuf, int ws) {
     if (fz > 0) {
         if (acc != 2 * V / 3) return nullptr;
-#define SHHD_K3FZ(nt, v, len)                                                                                  \
-    if (NT == nt && V == v && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 2> \
-                                                          : &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 1>;
+        if (ws == 4) {
+#define SHHD_K3WS(nt, v, len)                                                                                     \
+    if (NT == nt && V == v && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 2, 4> \
+                                                          : &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 1, 4>;
+            SHHD_FOR_COL_FZ_WS(SHHD_K3WS)
+#undef SHHD_K3WS
+            return nullptr;
+        }
+        if (ws != 1) return nullptr;
+#define SHHD_K3FZ(nt, v, len)                                                                                     \
+    if (NT == nt && V == v && fz == len) return nbuf == 2 ? &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 2, 1> \
+                                                          : &k_forward_columns<T, nt, v, (2 * (v) / 3), len, 1, 1>;
         SHHD_FOR_COL_FZ(SHHD_K3FZ)
 #undef SHHD_K3FZ
         return nullptr;
     }
+    if (ws != 1) return nullptr;
 #define SHHD_K3(nt, v) \
-    if (NT == nt && V == v && acc == 4) return &k_forward_columns<T, nt, v, 4, 0, 1>; \
-    if (NT == nt && V == v && acc == v) return &k_forward_columns<T, nt, v, v, 0, 1>;
+    if (NT == nt && V == v && acc == 4) return &k_forward_columns<T, nt, v, 4, 0, 1, 1>; \
+    if (NT == nt && V == v && acc == v) return &k_forward_columns<T, nt, v, v, 0, 1, 1>;
     SHHD_FOR_NT_V(SHHD_K3)
 #undef SHHD_K3
     return nullptr;
@@ -243,11 +271,21 @@ bool row_layout_built(int NT, int V, int maxn, int fl, int nbuf, int layout) {
 }
 
 template <typename T>
-K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf, int layout) {
+K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf, int layout, int ws) {
     const int n = maxn <= 0 ? 0 : (maxn == 1 ? 1 : (maxn == 2 ? 2 : (maxn <= 4 ? 4 : 8)));
+    if (ws == 4) {
+        if (fl != 0 || nbuf != 0 || layout != 0) return nullptr;
+#define SHHD_K2WN(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0, 0, 4>;
+#define SHHD_K2W(nt, v) SHHD_FOR_MAXN(SHHD_K2WN, nt, v)
+        SHHD_FOR_ROW_WS(SHHD_K2W)
+#undef SHHD_K2W
+#undef SHHD_K2WN
+        return nullptr;
+    }
+    if (ws != 1) return nullptr;
     if (fl > 0 && nbuf > 0 && layout > 0) {
 #define SHHD_K2LN(nt, v, len, nb, lay, mn) \
-    if (NT == nt && V == v && fl == len && nbuf == nb && layout == lay && n == mn) return &k_rows<T, nt, v, mn, len, nb, lay>;
+    if (NT == nt && V == v && fl == len && nbuf == nb && layout == lay && n == mn) return &k_rows<T, nt, v, mn, len, nb, lay, 1>;
 #define SHHD_K2L(nt, v, len) SHHD_FOR_LAYOUT(SHHD_K2LN, nt, v, len, 1) SHHD_FOR_LAYOUT(SHHD_K2LN, nt, v, len, 2)
         SHHD_FOR_ROW_FL(SHHD_K2L)
 #undef SHHD_K2L
@@ -256,14 +294,14 @@ K2Fn<T> pick_k2(int NT, int V, int maxn, int fl, int nbuf, int layout) {
     if (fl > 0) {
 #define SHHD_K2FN(nt, v, len, mn)                                                  \
     if (NT == nt && V == v && n == mn && fl == len)                                \
-        return nbuf == 2 ? &k_rows<T, nt, v, mn, len, 2, 0>                        \
-                         : (nbuf == 1 ? &k_rows<T, nt, v, mn, len, 1, 0> : &k_rows<T, nt, v, mn, len, 0, 0>);
+        return nbuf == 2 ? &k_rows<T, nt, v, mn, len, 2, 0, 1>                     \
+                         : (nbuf == 1 ? &k_rows<T, nt, v, mn, len, 1, 0, 1> : &k_rows<T, nt, v, mn, len, 0, 0, 1>);
 #define SHHD_K2F(nt, v, len) SHHD_FOR_MAXN(SHHD_K2FN, nt, v, len)
         SHHD_FOR_ROW_FL(SHHD_K2F)
 #undef SHHD_K2F
 #undef SHHD_K2FN
     }
-#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0, 0>;
+#define SHHD_K2N(nt, v, mn) if (NT == nt && V == v && n == mn) return &k_rows<T, nt, v, mn, 0, 0, 0, 1>;
 #define SHHD_K2(nt, v) SHHD_K2N(nt, v, 0) SHHD_K2N(nt, v, 1) SHHD_K2N(nt, v, 2) SHHD_K2N(nt, v, 4) SHHD_K2N(nt, v, 8)
     SHHD_FOR_NT_V(SHHD_K2)
 #undef SHHD_K2
@@ -853,6 +891,10 @@ struct Tuning {
     // blocks pay where the inverse column pass writes one narrow tile per
     // workgroup into long rows, e.g. FP32 at 4096^2)
     int tlg = -1;
+    // forward column pass and row pass: 0 wave groups where apply_plans'
+    // rule picks them, 1 never, 4 wherever a build exists
+    int k3ws = 0;
+    int k2ws = 0;
 };
 // launch shapes of a shape that was never tuned: default tiles and lanes,
 // the fused transforms with two LDS buffers where they fit (else one), U
@@ -949,6 +991,31 @@ void apply_plans(Instance& h, const Tuning& t) {
             p->nbuf = (fz_nb == 2 && two_buffers_fit(one, p->NT)) ? 2 : 1;
             p->lds = p->nbuf * one;
             p->lds3 = std::max(p->nbuf * one, static_cast<size_t>(2 * p->C * d.nay) * h.elem);
+        }
+    }
+    // A fused forward-column tile of one wave whose launch has few workgroups
+    // (fewer than half the CUs) runs as four wave groups that split its
+    // products: the transforms of a tile and its stage update then take a
+    // quarter of the rounds (small grids are bound by that chain, not by HBM).
+    const int kFewBlocks = 128;
+    {
+        RowPlan& r = h.rows;
+        const bool few = !h.batch_shape && r.nblocks <= kFewBlocks;
+        if (t.k2ws != 1 && r.NT == 64 && r.fl == 0 && r.V == 6 && row_ws_built(4 * r.NT, r.V) && (t.k2ws == 4 || few)) {
+            r.ws = 4;
+            r.NT = 4 * 64;
+            r.lds = 4 * static_cast<size_t>(lp_size(r.R * d.mx)) * h.elem +
+                    static_cast<size_t>(std::max(d.n_in, 1)) * r.R * d.mx * (h.elem / 2);   // transforms, physical fields
+        }
+    }
+    {
+        ColPlan& p = h.cols3;
+        const bool few = !h.batch_shape && p.ntiles * d.ngroups <= kFewBlocks;
+        if (t.k3ws != 1 && p.fz > 0 && p.NT == 64 && col_ws_built(4 * p.NT, p.V, p.fz) && (t.k3ws == 4 || few)) {
+            const size_t one = static_cast<size_t>(lp_size(p.C * d.my)) * h.elem;
+            p.ws = 4;
+            p.NT = 4 * 64;
+            p.lds3 = std::max(4 * p.nbuf * one, static_cast<size_t>(2 * 4 * p.C * d.nay) * h.elem);
         }
     }
     h.desc.ulg = (t.ulg > 0 && t.ulg <= 12 && d.hxa % (1 << t.ulg) == 0) ? t.ulg : 0;
@@ -1175,10 +1242,12 @@ struct ClassKey {
     bool fp32;
     int stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl;
     int cfz, cnb, c3fz, c3nb, rnb, rlay;   // fused builds (length, LDS buffers, field layout)
+    int c3ws, rws;                         // forward-column and row wave groups
     bool operator<(const ClassKey& o) const {
-        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl, cfz, cnb, c3fz, c3nb, rnb, rlay) <
+        return std::tie(fp32, stages, cNT, cV, c3NT, c3V, acc, rNT, rV, maxn, rfl, cfz, cnb, c3fz, c3nb, rnb, rlay, c3ws,
+                        rws) <
                std::tie(o.fp32, o.stages, o.cNT, o.cV, o.c3NT, o.c3V, o.acc, o.rNT, o.rV, o.maxn, o.rfl, o.cfz, o.cnb,
-                        o.c3fz, o.c3nb, o.rnb, o.rlay);
+                        o.c3fz, o.c3nb, o.rnb, o.rlay, o.c3ws, o.rws);
     }
 };
 
@@ -1922,9 +1991,9 @@ struct Engine {
         const ClassKey& K = lc.key;
         SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k1<T>(K.cNT, K.cV, K.cfz, K.cnb)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[0])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay, K.rws)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[1])));
-        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb)),
+        SHHD_HIP(&err, hipFuncSetAttribute(reinterpret_cast<const void*>(pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb, K.c3ws)),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lc.lds[2])));
         return SHHD_OK;
     }
@@ -1941,7 +2010,8 @@ struct Engine {
             if (!h || h->failed || h->held) continue;   // a stopped or held instance is in no launch
             ClassKey k{h->fp32,      h->stages,     h->cols.NT,     h->cols.V,       h->cols3.NT, h->cols3.V,
                        h->cols3.acc, h->rows.NT,    h->rows.V,      h->rows.maxn,    h->rows.fl,  h->cols.fz,
-                       h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf, h->rows.layout};
+                       h->cols.nbuf, h->cols3.fz,   h->cols3.nbuf,  h->rows.nbuf, h->rows.layout, h->cols3.ws,
+                       h->rows.ws};
             groups[k].push_back(s);
         }
         std::vector<int> active(kMaxSlots, 0);
@@ -2005,13 +2075,13 @@ struct Engine {
             if (st != SHHD_OK) return st;
         }
         if (lc.nblk[1] > 0) {
-            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay);
+            K2Fn<T> k2 = pick_k2<T>(K.rNT, K.rV, K.maxn, K.rfl, K.rnb, K.rlay, K.rws);
             st = timed(1, capturing, [&] {
                 hipLaunchKernelGGL(k2, dim3(lc.nblk[1]), dim3(K.rNT), lc.lds[1], ls, inst, lc.map[1]);
             });
             if (st != SHHD_OK) return st;
         }
-        K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb);
+        K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb, K.c3ws);
         return timed(2, capturing, [&] {
             hipLaunchKernelGGL(k3, dim3(lc.nblk[2]), dim3(K.c3NT), lc.lds[2], ls, inst, ptrs, sp, lc.map[2]);
         });
@@ -2384,6 +2454,25 @@ shhd_status autotune_config(const shhd_config& cfg, int steps, Tuning& best, dou
         if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
         if (ms[0] + ms[1] < bk) { bk = ms[0] + ms[1]; best.tlg = tlg; best_ms[0] = ms[0]; best_ms[1] = ms[1]; }
     }
+    cur.tlg = best.tlg;
+    // forward-column wave groups (one-wave fused tiles only; elsewhere both
+    // choices plan the same launch)
+    bk = 1e30;
+    for (int ws : {1, 4}) {
+        Tuning t = cur;
+        t.k3ws = ws;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[2] < bk) { bk = ms[2]; best.k3ws = ws; best_ms[2] = ms[2]; }
+    }
+    cur.k3ws = best.k3ws;
+    // row-pass wave groups (one-wave staged row tiles only)
+    bk = 1e30;
+    for (int ws : {1, 4}) {
+        Tuning t = cur;
+        t.k2ws = ws;
+        if ((st = e.replan(slot, t)) != SHHD_OK || (st = time_classes(e, steps, ms)) != SHHD_OK) return set_err(err, st, e.err);
+        if (ms[1] < bk) { bk = ms[1]; best.k2ws = ws; best_ms[1] = ms[1]; }
+    }
     store_tuning(h.tune_key, best);
     return SHHD_OK;
 }
@@ -2472,6 +2561,8 @@ shhd_status shhd_autotune(const shhd_config* config, int steps, shhd_tuning* out
     out->row_transform = best.row_transform;
     out->plane_block = best.ulg;
     out->inverse_plane_block = best.tlg + 1;
+    out->forward_column_waves = best.k3ws;
+    out->row_waves = best.k2ws;
     for (int k = 0; k < kClasses; ++k) out->ms[k] = ms[k];
     return SHHD_OK;
 }
@@ -2486,10 +2577,13 @@ shhd_status shhd_set_tuning(const shhd_config* config, const shhd_tuning* t) {
         (t->row_lanes != 0 && t->row_lanes != 64 && t->row_lanes != 256 && t->row_lanes != 512 && t->row_lanes != 1024) ||
         t->column_transform < 0 || t->column_transform >= kTransforms || t->row_transform < 0 ||
         t->row_transform >= kTransforms || t->plane_block < 0 || t->plane_block > 12 || t->inverse_plane_block < 0 ||
-        t->inverse_plane_block > 13)
+        t->inverse_plane_block > 13 ||
+        (t->forward_column_waves != 0 && t->forward_column_waves != 1 && t->forward_column_waves != 4) ||
+        (t->row_waves != 0 && t->row_waves != 1 && t->row_waves != 4))
         return set_err(nullptr, SHHD_ERR_INVALID, "invalid launch shapes");
     store_tuning(h->tune_key, Tuning{t->inverse_column_tile, t->forward_column_tile, t->row_lanes, t->column_transform,
-                                     t->row_transform, t->plane_block, t->inverse_plane_block - 1});
+                                     t->row_transform, t->plane_block, t->inverse_plane_block - 1,
+                                     t->forward_column_waves, t->row_waves});
     return SHHD_OK;
 }
 

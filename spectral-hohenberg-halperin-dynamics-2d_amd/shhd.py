@@ -54,7 +54,8 @@ PROBE_STATE, PROBE_FLUX = 0, 1
 class ShhdTuning(ctypes.Structure):
     _fields_ = [("inverse_column_tile", ctypes.c_int), ("forward_column_tile", ctypes.c_int),
                 ("row_lanes", ctypes.c_int), ("column_transform", ctypes.c_int), ("row_transform", ctypes.c_int),
-                ("plane_block", ctypes.c_int), ("inverse_plane_block", ctypes.c_int), ("ms", ctypes.c_double * 3)]
+                ("plane_block", ctypes.c_int), ("inverse_plane_block", ctypes.c_int),
+                ("forward_column_waves", ctypes.c_int), ("row_waves", ctypes.c_int), ("ms", ctypes.c_double * 3)]
 
 
 class ShhdInitialCondition(ctypes.Structure):
@@ -724,17 +725,19 @@ def autotune(cfg: SolverConfig, steps: int = 6) -> dict:
         raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())
     return {"inverse_column_tile": t.inverse_column_tile, "forward_column_tile": t.forward_column_tile,
             "row_lanes": t.row_lanes, "column_transform": t.column_transform, "row_transform": t.row_transform,
-            "plane_block": t.plane_block, "inverse_plane_block": t.inverse_plane_block, "ms": list(t.ms)}
+            "plane_block": t.plane_block, "inverse_plane_block": t.inverse_plane_block,
+            "forward_column_waves": t.forward_column_waves, "row_waves": t.row_waves, "ms": list(t.ms)}
 
 
 def set_launch_shapes(cfg: SolverConfig, inverse_column_tile: int, forward_column_tile: int, row_lanes: int,
                       column_transform: int = 0, row_transform: int = 0, plane_block: int = 0,
-                      inverse_plane_block: int = 0):
+                      inverse_plane_block: int = 0, forward_column_waves: int = 0, row_waves: int = 0):
     """Record launch shapes measured by an earlier autotune (shhd_set_tuning)."""
     lib = load_library()
     c = cfg.to_c()
     t = ShhdTuning(int(inverse_column_tile), int(forward_column_tile), int(row_lanes), int(column_transform),
-                   int(row_transform), int(plane_block), int(inverse_plane_block))
+                   int(row_transform), int(plane_block), int(inverse_plane_block), int(forward_column_waves),
+                   int(row_waves))
     st = lib.shhd_set_tuning(ctypes.byref(c), ctypes.byref(t))
     if st != 0:
         raise ShhdError(st, lib.shhd_gpu_last_error(None).decode())

@@ -151,3 +151,46 @@ def test_inverse_plane_blocks_give_identical_results(fp32, grid):
             for k in range(3):
                 assert np.array_equal(out[k], ref[k]), (ct, tb, k)
 
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("na", [64, 128, 256])
+def test_wave_groups_match_one_wave_tiles(fp32, na):
+    """Short columns and rows on few workgroups split their transforms over
+    four wave groups (forward_column_waves / row_waves = 4).  The row pass
+    then computes exactly what one wave computes (bit for bit); the forward
+    column pass adds the groups' partial sums of each mode, so the states
+    agree to rounding, and with the step flux recorded (group 0 takes every
+    product) bit for bit, fluxes included."""
+    print(f"[case] wave groups na={na} fp32={fp32}", flush=True)
+    cfg = model_h(na, na, fp32=fp32)
+    st = flux_state(cfg)
+    tol = 1e-5 if fp32 else 1e-12
+
+    def go(fw, rw, flux):
+        shhd.set_launch_shapes(cfg, 0, 0, 0, 3, 3, 1, 0, fw, rw)
+        with shhd.GpuFluidSolver(cfg) as g:
+            if flux:
+                g.set_flux_request(density=True, order_parameter=True, momentum=True)
+            g.set_state(st)
+            g.step(4)
+            return g.get_state(), (g.get_flux() if flux else None), g.info()
+
+    for flux in (False, True):
+        ref = go(1, 1, flux)
+        assert ref[2].row_lanes == 64   # one-wave row tiles at these lengths
+        # the forward column pass splits fused one-wave tiles only (192-point
+        # columns in both precisions; 384-point FP64 columns)
+        split3 = ref[2].forward_column_fused > 0 and ref[2].forward_column_lanes == 64
+        assert split3 or na != 128
+        for fw, rw in ((4, 1), (1, 4), (4, 4)):
+            out = go(fw, rw, flux)
+            assert out[2].forward_column_lanes == (256 if fw == 4 and split3 else ref[2].forward_column_lanes)
+            assert out[2].row_lanes == (256 if rw == 4 else 64)
+            if flux or fw == 1:
+                assert np.array_equal(out[0], ref[0]), (fw, rw, flux)
+                if flux:
+                    assert np.array_equal(out[1], ref[1]), (fw, rw)
+            else:
+                for f in range(cfg.num_fields):
+                    assert rel_l2(out[0][f], ref[0][f]) <= tol, (fw, rw, f)
