@@ -452,7 +452,6 @@ template <typename T, int NT, int L, int NSEQ, int NBUF>
 __device__ __forceinline__ void fz_first_pair(FzBufs<T, NBUF>& bf, cx<T> (&v)[2][3], const cx<T>* __restrict__ tw,
                                               int sign, int tid) {
     static_assert(fz_pair_ok<L, true>() && NT * 6 == NSEQ * L, "radix-3 / radix-8 pair shape");
-    constexpr int K = L / 24;
     asm volatile("" : "+v"(tid));
     const FzGrp<L> G(tid);
     const bool act = G.r < 3;
