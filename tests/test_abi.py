@@ -39,6 +39,37 @@ def test_config_struct_layout_matches_header():
     assert shhd.ShhdConfig.seed.offset % 8 == 0
 
 
+BACKEND = os.path.join(ROOT, "spectral-hohenberg-halperin-dynamics-2d_amd", "host", "backend_check")
+
+
+def backend_check():
+    """The C++ driver of host/gpu_backend.hpp (built by the package Makefile;
+    built here when missing: host code only)."""
+    if not os.path.exists(BACKEND):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.dirname(os.path.dirname(BACKEND)), "host/backend_check"],
+                       check=True)
+    return BACKEND
+
+
+def test_ctypes_structs_match_the_compiled_header():
+    """Sizes and offsets of the C-ABI structs as a C++ compiler lays them out
+    (the backend driver, compiled against include/shhd_gpu.h) equal the
+    ctypes mirrors of the Python binding."""
+    import subprocess
+    out = subprocess.run([backend_check(), "layout"], check=True, capture_output=True, text=True).stdout
+    got = dict(line.split() for line in out.strip().splitlines())
+    types = {"shhd_config": shhd.ShhdConfig, "shhd_sine_force": shhd.ShhdSineForce,
+             "shhd_gradient_force": shhd.ShhdGradientForce, "shhd_info": shhd.ShhdInfo, "shhd_probe": shhd.ShhdProbe,
+             "shhd_initial_condition": shhd.ShhdInitialCondition, "shhd_tuning": shhd.ShhdTuning}
+    assert len(got) >= 19
+    for key, val in got.items():
+        name, _, field = key.partition(".")
+        t = types[name]
+        want = getattr(t, field).offset if field else ctypes.sizeof(t)
+        assert int(val) == want, (key, val, want)
+
+
 @pytest.mark.parametrize("kw,msg", [
     (dict(nx_active=7, dealias="none"), "grid must be even"),
     (dict(mode=shhd.MODE_QUIESCENT, momentum_advection=True), "quiescent time evolution cannot use momentum"),
