@@ -115,9 +115,19 @@ __device__ __forceinline__ int fz_slot(int i) { return PAD ? lp(i) : i; }
 
 // Layout of the buffer pass P writes: plain after a radix-3 first pass (its
 // stride-3 writes are conflict-free unpadded and conflict 2-way padded; the
-// next pass reads runs of consecutive elements, conflict-free either way),
-// padded after every other pass (stride 8, 4, 2 writes).
-constexpr bool fz_pad_after(int L, bool r3_first, int P) { return !(P == 0 && fz_radix(L, r3_first, 0) == 3); }
+// next pass reads runs of consecutive elements, conflict-free either way) and
+// after an inner pass of sub-length 24 (its writes are runs of 24 consecutive
+// elements over consecutive lanes: conflict-free unpadded for 16-byte
+// elements, at most 2-way for 8-byte ones, where the padded slot would cost
+// four integer operations per output), padded after every other pass
+// (stride 8, 4, 2 writes) and after the last pass (the callers read the
+// result in the padded layout).
+constexpr bool fz_pad_after(int L, bool r3_first, int P) {
+    return !(P == 0 && fz_radix(L, r3_first, 0) == 3) &&
+           !(fz_ns(L, r3_first, P) == 24 && P + 1 < fz_npass(L, r3_first));
+}
+
+constexpr bool fz_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // Stockham outputs of the butterflies in v to LDS.
 template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
@@ -131,10 +141,15 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
         const int j = Mp::bfly(id);
         const int k = j % NS;
         const int o = s * L + (j - k) * R + k;
-        if constexpr (PAD && NS % kPad == 0) {
+        // With NS and R powers of two (and every sequence starting on a pad
+        // block), lp(o + p NS) = lp(o) + p NS + (p NS >> kPadShift) for every
+        // butterfly: o = (multiple of NS R) + k with k < NS, so no output run
+        // crosses a pad slot it does not account for; the per-output offsets
+        // are then compile-time constants.
+        if constexpr (PAD && (NS % kPad == 0 || (fz_pow2(NS) && fz_pow2(R) && (NSEQ == 1 || L % kPad == 0)))) {
             const int o0 = lp(o);
 #pragma unroll
-            for (int p = 0; p < R; ++p) out[o0 + p * (NS + NS / kPad)] = v[b][p];
+            for (int p = 0; p < R; ++p) out[o0 + p * NS + ((p * NS) >> kPadShift)] = v[b][p];
         } else {
 #pragma unroll
             for (int p = 0; p < R; ++p) out[fz_slot<PAD>(o + p * NS)] = v[b][p];
