@@ -1286,215 +1286,256 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     // the pointers and coefficients of its own fields
     auto stage_modes = [&](auto psi_tag) {
         constexpr bool kPsi = decltype(psi_tag)::value;
-        for (int m = tid; m < nmodes; m += NT) {
-            const int c = divn(m, nay, d.lnay);
-            const int cidx = m - c * nay;
-            const int gx = gx0 + c;
-            if (gx >= hxa) continue;
-            const uint32_t at = static_cast<uint32_t>(gx * nay + cidx);
-            if (cidx == half) {   // the always-zero ky = -Nay/2 slot
-                if constexpr (kPsi) {
-                    gstore(P.next[1 + a], at, czero<T>());
-                } else {
-                    if (compressible) gstore(P.next[0], at, czero<T>());
-                    gstore(P.next[fx_field], at, czero<T>());
-                    gstore(P.next[fy_field], at, czero<T>());
+        // The state loads of PF modes of the lane (FP64 below 1024 lanes:
+        // two; the modes m = tid + NT i, i < ACC, host-checked) are issued
+        // together before any of them is updated; a slot the lane does not
+        // own loads mode 0 of the field and is not used.  The base state is
+        // read only at the stages that weight it.  (FP32 and the 1024-lane
+        // builds, capped at 128 registers, keep one mode at a time: the extra
+        // registers measured slower there.)
+        constexpr int NF = kPsi ? 1 : 3;   // psi_a | rho, jx, jy
+        constexpr int PF = (sizeof(T) == 8 && NT < 1024 && ACC >= 2) ? 2 : 1;
+        const bool use_base = sp.wb != 0.0;
+        const bool need_rho = !kPsi && (compressible || (d.mom_det_on && d.pressure_coefficient != 0.0));
+        auto field_of = [&](int q) { return kPsi ? 1 + a : (q == 0 ? 0 : (q == 1 ? fx_field : fy_field)); };
+        for (int i0 = 0; i0 < ACC && tid + NT * i0 < nmodes; i0 += PF) {
+            cx<T> pcur[PF][NF], pbas[PF][NF];
+#pragma unroll
+            for (int ii = 0; ii < PF; ++ii) {
+                const int m = tid + NT * (i0 + ii);
+                const int c = divn(m, nay, d.lnay);
+                const bool own = i0 + ii < ACC && m < nmodes && gx0 + c < hxa;
+                const uint32_t at = own ? static_cast<uint32_t>((gx0 + c) * nay + (m - c * nay)) : 0u;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    pcur[ii][q] = (q > 0 || kPsi || need_rho) ? gload(P.cur[field_of(q)], at) : czero<T>();
+                    pbas[ii][q] = czero<T>();
                 }
-                continue;
             }
-            const int ky_i = cidx < half ? cidx : cidx - nay;
-            const T kx = dkx * T(gx);
-            const T ky = dky * T(ky_i);
-            const T k2 = kx * kx + ky * ky;
-            const bool is_k0 = kFp32 && gx == 0 && cidx == 0 && P.next_k0 != nullptr;
-            // canonical mode of the noise draw: the kx = 0 line draws the ky > 0
-            // member of each pair and conjugates it onto ky < 0
-            // (src/fcalculator.cc:525-566)
-            const bool conj_draw = (gx == 0 && ky_i < 0);
-            const uint32_t mode_id = static_cast<uint32_t>(gx * nay + (conj_draw ? -ky_i : cidx));
+            if (use_base) {
+#pragma unroll
+                for (int ii = 0; ii < PF; ++ii) {
+                    const int m = tid + NT * (i0 + ii);
+                    const int c = divn(m, nay, d.lnay);
+                    const bool own = i0 + ii < ACC && m < nmodes && gx0 + c < hxa;
+                    const uint32_t at = own ? static_cast<uint32_t>((gx0 + c) * nay + (m - c * nay)) : 0u;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        if (q > 0 || kPsi || compressible) pbas[ii][q] = gload(P.base[field_of(q)], at);
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < PF; ++ii) {
+                const int m = tid + NT * (i0 + ii);
+                if (i0 + ii >= ACC || m >= nmodes) break;
+                const int c = divn(m, nay, d.lnay);
+                const int cidx = m - c * nay;
+                const int gx = gx0 + c;
+                if (gx >= hxa) continue;
+                const uint32_t at = static_cast<uint32_t>(gx * nay + cidx);
+                if (cidx == half) {   // the always-zero ky = -Nay/2 slot
+                    if constexpr (kPsi) {
+                        gstore(P.next[1 + a], at, czero<T>());
+                    } else {
+                        if (compressible) gstore(P.next[0], at, czero<T>());
+                        gstore(P.next[fx_field], at, czero<T>());
+                        gstore(P.next[fy_field], at, czero<T>());
+                    }
+                    continue;
+                }
+                const int ky_i = cidx < half ? cidx : cidx - nay;
+                const T kx = dkx * T(gx);
+                const T ky = dky * T(ky_i);
+                const T k2 = kx * kx + ky * ky;
+                const bool is_k0 = kFp32 && gx == 0 && cidx == 0 && P.next_k0 != nullptr;
+                // canonical mode of the noise draw: the kx = 0 line draws the ky > 0
+                // member of each pair and conjugates it onto ky < 0
+                // (src/fcalculator.cc:525-566)
+                const bool conj_draw = (gx == 0 && ky_i < 0);
+                const uint32_t mode_id = static_cast<uint32_t>(gx * nay + (conj_draw ? -ky_i : cidx));
 
-            // every load of the mode is issued here, ahead of the noise draws and
-            // the flux updates (whose stores the later loads could not pass);
-            // the base state is read only when its weight is nonzero
-            const bool use_base = wb != T(0);
-            if constexpr (kPsi) {
-                const int f = 1 + a;
-                const cx<T> cur = gload(P.cur[f], at);
-                const cx<T> bas = use_base ? gload(P.base[f], at) : czero<T>();
-                cx<T> det = acc_sum(acc_x, m);
-                if (d.psi_det_on[a]) {
-                    if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
-                        const T mob = T(d.mobility[a]);
-                        const cx<T> mu = (T(d.mu_k0[a]) + T(d.mu_k2[a]) * k2 + T(d.mu_k4[a]) * k2 * k2) * cur;
-                        const cx<T> fx = mul_mi((mob * kx) * mu);
-                        const cx<T> fy = mul_mi((mob * ky) * mu);
-                        det = det + mul_mi(kx * fx + ky * fy);
-                        if (fx_slot) {
-                            fx_slot[at] = fx_slot[at] + fw * fx;
-                            fy_slot[at] = fy_slot[at] + fw * fy;
+                // every load of the mode is issued here, ahead of the noise draws and
+                // the flux updates (whose stores the later loads could not pass);
+                // the base state is read only when its weight is nonzero
+                if constexpr (kPsi) {
+                    const int f = 1 + a;
+                    const cx<T> cur = pcur[ii][0];
+                    const cx<T> bas = pbas[ii][0];
+                    cx<T> det = acc_sum(acc_x, m);
+                    if (d.psi_det_on[a]) {
+                        if (d.psi_lin_on[a]) {   // src/fcalculator.cc:152-187
+                            const T mob = T(d.mobility[a]);
+                            const cx<T> mu = (T(d.mu_k0[a]) + T(d.mu_k2[a]) * k2 + T(d.mu_k4[a]) * k2 * k2) * cur;
+                            const cx<T> fx = mul_mi((mob * kx) * mu);
+                            const cx<T> fy = mul_mi((mob * ky) * mu);
+                            det = det + mul_mi(kx * fx + ky * fy);
+                            if (fx_slot) {
+                                fx_slot[at] = fx_slot[at] + fw * fx;
+                                fy_slot[at] = fy_slot[at] + fw * fy;
+                            }
+                        }
+                        for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
+                            if (d.igrad_component[q] != a) continue;
+                            const cx<T> j = gload(P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field], at);
+                            const cx<T> t = T(-d.igrad_amplitude[q]) * j;
+                            det = det + mk(t.re / T(d.rho0), t.im / T(d.rho0));
+                        }
+                        for (int q = 0; q < d.nsources; ++q) {
+                            if (d.sources[q].field == f && d.sources[q].gx == gx && d.sources[q].cidx == cidx)
+                                det = det + mk(T(d.sources[q].re), T(d.sources[q].im));
                         }
                     }
-                    for (int q = 0; q < d.n_igrad; ++q) {   // src/fcalculator.cc:797-812
-                        if (d.igrad_component[q] != a) continue;
-                        const cx<T> j = gload(P.cur[d.igrad_direction[q] == 0 ? fx_field : fy_field], at);
-                        const cx<T> t = T(-d.igrad_amplitude[q]) * j;
-                        det = det + mk(t.re / T(d.rho0), t.im / T(d.rho0));
-                    }
-                    for (int q = 0; q < d.nsources; ++q) {
-                        if (d.sources[q].field == f && d.sources[q].gx == gx && d.sources[q].cidx == cidx)
-                            det = det + mk(T(d.sources[q].re), T(d.sources[q].im));
-                    }
-                }
-                cx<T> sto = czero<T>();
-                if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
-                    // draws A and B (conjugated on the kx = 0, ky < 0 half line);
-                    // SRK3 draws them once per step: stage 1 stores them, stages 2
-                    // and 3 read them back
-                    cx<T> sa, sb = czero<T>();
-                    cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
-                    if (sp.noise_mode == NOISE_LOAD) {
-                        sa = gload(nz, at);
-                        sb = gload(nz, fe32 + at);
-                    } else {
-                        const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
-                        gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
-                        sa = sigma * sa;
-                        sb = sp.use_b ? sigma * sb : czero<T>();
-                        if (conj_draw) { sa = conj(sa); sb = conj(sb); }
-                        if (sp.noise_mode == NOISE_STORE) {
-                            gstore(nz, at, sa);
-                            gstore(nz, fe32 + at, sb);
+                    cx<T> sto = czero<T>();
+                    if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
+                        // draws A and B (conjugated on the kx = 0, ky < 0 half line);
+                        // SRK3 draws them once per step: stage 1 stores them, stages 2
+                        // and 3 read them back
+                        cx<T> sa, sb = czero<T>();
+                        cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
+                        if (sp.noise_mode == NOISE_LOAD) {
+                            sa = gload(nz, at);
+                            sb = gload(nz, fe32 + at);
+                        } else {
+                            const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
+                            gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
+                            sa = sigma * sa;
+                            sb = sp.use_b ? sigma * sb : czero<T>();
+                            if (conj_draw) { sa = conj(sa); sb = conj(sb); }
+                            if (sp.noise_mode == NOISE_STORE) {
+                                gstore(nz, at, sa);
+                                gstore(nz, fe32 + at, sb);
+                            }
                         }
-                    }
-                    sto = sp.use_b ? sa + beta * sb : sa;
-                    if (fx_slot && sp.flux_first) {   // draw A: (i k / k^2) sigma xi, times 1/sqrt(dt)
-                        const T isd = T(1.0 / d.sqrt_dt);
-                        fx_slot[at] = fx_slot[at] + isd * mul_i((kx / k2) * sa);
-                        fy_slot[at] = fy_slot[at] + isd * mul_i((ky / k2) * sa);
-                    }
-                }
-                if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
-                else gstore(P.next[f], at, wb * bas + wc * (cur + dt * det + sdt * sto));
-            } else {
-                const bool pressure = d.mom_det_on && d.pressure_coefficient != 0.0;
-                const cx<T> jx = gload(P.cur[fx_field], at);
-                const cx<T> jy = gload(P.cur[fy_field], at);
-                const cx<T> rho = (compressible || pressure) ? gload(P.cur[0], at) : czero<T>();
-                const cx<T> bx = use_base ? gload(P.base[fx_field], at) : czero<T>();
-                const cx<T> by = use_base ? gload(P.base[fy_field], at) : czero<T>();
-                const cx<T> brho = (use_base && compressible) ? gload(P.base[0], at) : czero<T>();
-                cx<T> dx = acc_sum(acc_x, m), dy = acc_sum(acc_y, m);
-                if (d.mom_det_on) {
-                    if (pressure) {   // src/fcalculator.cc:276-300
-                        const cx<T> p = T(d.pressure_coefficient) * rho;
-                        dx = dx + mul_mi(kx * p);
-                        dy = dy + mul_mi(ky * p);
-                        if (pi_slot) {
-                            pi_slot[at] = pi_slot[at] + fw * p;
-                            pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * p;
-                        }
-                    }
-                    if (d.incompressible_viscous) {   // src/fcalculator.cc:302-341
-                        const T eta = T(d.eta);
-                        const T r0 = T(d.rho0);
-                        const cx<T> vx = mk(jx.re / r0, jx.im / r0);
-                        const cx<T> vy = mk(jy.re / r0, jy.im / r0);
-                        const cx<T> pxx = mul_mi(eta * ((T(2) * kx) * vx));
-                        const cx<T> pxy = mul_mi(eta * (ky * vx + kx * vy));
-                        const cx<T> pyy = mul_mi(eta * ((T(2) * ky) * vy));
-                        dx = dx + mul_mi(kx * pxx + ky * pxy);
-                        dy = dy + mul_mi(kx * pxy + ky * pyy);
-                        if (pi_slot) {
-                            pi_slot[at] = pi_slot[at] + fw * pxx;
-                            pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
-                            pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
-                        }
-                    }
-                    for (int q = 0; q < d.nsources; ++q) {
-                        if (d.sources[q].gx != gx || d.sources[q].cidx != cidx) continue;
-                        const cx<T> sv = mk(T(d.sources[q].re), T(d.sources[q].im));
-                        if (d.sources[q].field == fx_field) dx = dx + sv;
-                        if (d.sources[q].field == fy_field) dy = dy + sv;
-                    }
-                }
-                cx<T> sx = czero<T>(), sy = czero<T>();
-                cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
-                if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
-                    sx = gload(nzj, at) + beta * gload(nzj, 2 * fe32 + at);
-                    sy = gload(nzj, fe32 + at) + beta * gload(nzj, 3 * fe32 + at);
-                } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
-                    const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
-                    const T kxd = conj_draw ? T(0) : kx;
-                    const T kyd = conj_draw ? -ky : ky;
-                    const int ndraw = sp.use_b ? 2 : 1;
-                    cx<T> x3a, x3b;   // the xy deviates of draws A and B
-                    gauss_quad<T>(d, mode_id, 10u, sp, x3a, x3b);
-                    for (int ab = 0; ab < ndraw; ++ab) {
-                        cx<T> x1, x2;
-                        gauss_quad<T>(d, mode_id, 8u + static_cast<uint32_t>(ab), sp, x1, x2);
-                        const cx<T> x3 = ab == 0 ? x3a : x3b;
-                        const cx<T> pxx = -ss * (sz * x1 + se * x2);
-                        const cx<T> pyy = -ss * (sz * x1 - se * x2);
-                        const cx<T> pxy = (-ss * se) * x3;
-                        cx<T> vx = mul_mi(kxd * pxx + kyd * pxy);
-                        cx<T> vy = mul_mi(kxd * pxy + kyd * pyy);
-                        if (conj_draw) { vx = conj(vx); vy = conj(vy); }
-                        if (ab == 0 && pi_slot && sp.flux_first) {   // draw A stress, times 1/sqrt(dt)
+                        sto = sp.use_b ? sa + beta * sb : sa;
+                        if (fx_slot && sp.flux_first) {   // draw A: (i k / k^2) sigma xi, times 1/sqrt(dt)
                             const T isd = T(1.0 / d.sqrt_dt);
-                            pi_slot[at] = pi_slot[at] + isd * (conj_draw ? conj(pxx) : pxx);
-                            pi_slot[at + fe] = pi_slot[at + fe] + isd * (conj_draw ? conj(pxy) : pxy);
-                            pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
+                            fx_slot[at] = fx_slot[at] + isd * mul_i((kx / k2) * sa);
+                            fy_slot[at] = fy_slot[at] + isd * mul_i((ky / k2) * sa);
                         }
-                        if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
-                            gstore(nzj, (2 * ab) * fe32 + at, vx);
-                            gstore(nzj, (2 * ab + 1) * fe32 + at, vy);
+                    }
+                    if (is_k0) k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[f] + at, f, dt * det, sp);
+                    else gstore(P.next[f], at, wb * bas + wc * (cur + dt * det + sdt * sto));
+                } else {
+                    const bool pressure = d.mom_det_on && d.pressure_coefficient != 0.0;
+                    const cx<T> jx = pcur[ii][1];
+                    const cx<T> jy = pcur[ii][2];
+                    const cx<T> rho = pcur[ii][0];
+                    const cx<T> bx = pbas[ii][1];
+                    const cx<T> by = pbas[ii][2];
+                    const cx<T> brho = pbas[ii][0];
+                    cx<T> dx = acc_sum(acc_x, m), dy = acc_sum(acc_y, m);
+                    if (d.mom_det_on) {
+                        if (pressure) {   // src/fcalculator.cc:276-300
+                            const cx<T> p = T(d.pressure_coefficient) * rho;
+                            dx = dx + mul_mi(kx * p);
+                            dy = dy + mul_mi(ky * p);
+                            if (pi_slot) {
+                                pi_slot[at] = pi_slot[at] + fw * p;
+                                pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * p;
+                            }
                         }
-                        if (ab == 0) { sx = vx; sy = vy; }
-                        else { sx = sx + beta * vx; sy = sy + beta * vy; }
+                        if (d.incompressible_viscous) {   // src/fcalculator.cc:302-341
+                            const T eta = T(d.eta);
+                            const T r0 = T(d.rho0);
+                            const cx<T> vx = mk(jx.re / r0, jx.im / r0);
+                            const cx<T> vy = mk(jy.re / r0, jy.im / r0);
+                            const cx<T> pxx = mul_mi(eta * ((T(2) * kx) * vx));
+                            const cx<T> pxy = mul_mi(eta * (ky * vx + kx * vy));
+                            const cx<T> pyy = mul_mi(eta * ((T(2) * ky) * vy));
+                            dx = dx + mul_mi(kx * pxx + ky * pxy);
+                            dy = dy + mul_mi(kx * pxy + ky * pyy);
+                            if (pi_slot) {
+                                pi_slot[at] = pi_slot[at] + fw * pxx;
+                                pi_slot[at + fe] = pi_slot[at + fe] + fw * pxy;
+                                pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + fw * pyy;
+                            }
+                        }
+                        for (int q = 0; q < d.nsources; ++q) {
+                            if (d.sources[q].gx != gx || d.sources[q].cidx != cidx) continue;
+                            const cx<T> sv = mk(T(d.sources[q].re), T(d.sources[q].im));
+                            if (d.sources[q].field == fx_field) dx = dx + sv;
+                            if (d.sources[q].field == fy_field) dy = dy + sv;
+                        }
                     }
-                }
-                if (pi_slot && sp.flux_last && !compressible && k2 != T(0)) {
-                    // FluxBuffer::add_incompressible_projection_pressure_flux
-                    // (src/buffer_flux.cc:196-225) on the step's accumulated stress
-                    const cx<T> txx = pi_slot[at], txy = pi_slot[at + fe], tyy = pi_slot[at + 2 * fe];
-                    const cx<T> rx = mul_mi(kx * txx + ky * txy);
-                    const cx<T> ry = mul_mi(kx * txy + ky * tyy);
-                    const cx<T> kr = kx * rx + ky * ry;
-                    const cx<T> pk = mul_mi(mk(kr.re / k2, kr.im / k2));
-                    pi_slot[at] = txx + pk;
-                    pi_slot[at + 2 * fe] = tyy + pk;
-                }
-                if (compressible) {
-                    const cx<T> drho = mul_mi(kx * jx + ky * jy);   // src/fcalculator.cc:60-84
-                    if (fx_slot) {
-                        fx_slot[at] = fx_slot[at] + fw * jx;
-                        fy_slot[at] = fy_slot[at] + fw * jy;
+                    cx<T> sx = czero<T>(), sy = czero<T>();
+                    cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
+                    if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
+                        sx = gload(nzj, at) + beta * gload(nzj, 2 * fe32 + at);
+                        sy = gload(nzj, fe32 + at) + beta * gload(nzj, 3 * fe32 + at);
+                    } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
+                        const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
+                        const T kxd = conj_draw ? T(0) : kx;
+                        const T kyd = conj_draw ? -ky : ky;
+                        const int ndraw = sp.use_b ? 2 : 1;
+                        cx<T> x3a, x3b;   // the xy deviates of draws A and B
+                        gauss_quad<T>(d, mode_id, 10u, sp, x3a, x3b);
+                        for (int ab = 0; ab < ndraw; ++ab) {
+                            cx<T> x1, x2;
+                            gauss_quad<T>(d, mode_id, 8u + static_cast<uint32_t>(ab), sp, x1, x2);
+                            const cx<T> x3 = ab == 0 ? x3a : x3b;
+                            const cx<T> pxx = -ss * (sz * x1 + se * x2);
+                            const cx<T> pyy = -ss * (sz * x1 - se * x2);
+                            const cx<T> pxy = (-ss * se) * x3;
+                            cx<T> vx = mul_mi(kxd * pxx + kyd * pxy);
+                            cx<T> vy = mul_mi(kxd * pxy + kyd * pyy);
+                            if (conj_draw) { vx = conj(vx); vy = conj(vy); }
+                            if (ab == 0 && pi_slot && sp.flux_first) {   // draw A stress, times 1/sqrt(dt)
+                                const T isd = T(1.0 / d.sqrt_dt);
+                                pi_slot[at] = pi_slot[at] + isd * (conj_draw ? conj(pxx) : pxx);
+                                pi_slot[at + fe] = pi_slot[at + fe] + isd * (conj_draw ? conj(pxy) : pxy);
+                                pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
+                            }
+                            if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
+                                gstore(nzj, (2 * ab) * fe32 + at, vx);
+                                gstore(nzj, (2 * ab + 1) * fe32 + at, vy);
+                            }
+                            if (ab == 0) { sx = vx; sy = vy; }
+                            else { sx = sx + beta * vx; sy = sy + beta * vy; }
+                        }
                     }
-                    if (is_k0) {
-                        k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[0] + at, 0, dt * drho, sp);
-                        k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
-                        k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
-                    } else {
-                        gstore(P.next[0], at, wb * brho + wc * (rho + dt * drho));
-                        gstore(P.next[fx_field], at, wb * bx + wc * (jx + dt * dx + sdt * sx));
-                        gstore(P.next[fy_field], at, wb * by + wc * (jy + dt * dy + sdt * sy));
+                    if (pi_slot && sp.flux_last && !compressible && k2 != T(0)) {
+                        // FluxBuffer::add_incompressible_projection_pressure_flux
+                        // (src/buffer_flux.cc:196-225) on the step's accumulated stress
+                        const cx<T> txx = pi_slot[at], txy = pi_slot[at + fe], tyy = pi_slot[at + 2 * fe];
+                        const cx<T> rx = mul_mi(kx * txx + ky * txy);
+                        const cx<T> ry = mul_mi(kx * txy + ky * tyy);
+                        const cx<T> kr = kx * rx + ky * ry;
+                        const cx<T> pk = mul_mi(mk(kr.re / k2, kr.im / k2));
+                        pi_slot[at] = txx + pk;
+                        pi_slot[at + 2 * fe] = tyy + pk;
                     }
-                } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
-                    cx<T> ix = dt * dx + sdt * sx;
-                    cx<T> iy = dt * dy + sdt * sy;
-                    if (k2 != T(0)) {
-                        const cx<T> tr = (-ky) * ix + kx * iy;
-                        const cx<T> px = (-ky) * tr;
-                        const cx<T> py = kx * tr;
-                        ix = mk(px.re / k2, px.im / k2);
-                        iy = mk(py.re / k2, py.im / k2);
-                    }
-                    if (is_k0) {
-                        k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
-                        k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
-                    } else {
-                        gstore(P.next[fx_field], at, wb * bx + wc * (jx + ix));
-                        gstore(P.next[fy_field], at, wb * by + wc * (jy + iy));
+                    if (compressible) {
+                        const cx<T> drho = mul_mi(kx * jx + ky * jy);   // src/fcalculator.cc:60-84
+                        if (fx_slot) {
+                            fx_slot[at] = fx_slot[at] + fw * jx;
+                            fy_slot[at] = fy_slot[at] + fw * jy;
+                        }
+                        if (is_k0) {
+                            k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[0] + at, 0, dt * drho, sp);
+                            k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, dt * dx, sp);
+                            k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, dt * dy, sp);
+                        } else {
+                            gstore(P.next[0], at, wb * brho + wc * (rho + dt * drho));
+                            gstore(P.next[fx_field], at, wb * bx + wc * (jx + dt * dx + sdt * sx));
+                            gstore(P.next[fy_field], at, wb * by + wc * (jy + dt * dy + sdt * sy));
+                        }
+                    } else {   // incompressible: projected increment (time_integrator_srk3_incompressible.h:81-107)
+                        cx<T> ix = dt * dx + sdt * sx;
+                        cx<T> iy = dt * dy + sdt * sy;
+                        if (k2 != T(0)) {
+                            const cx<T> tr = (-ky) * ix + kx * iy;
+                            const cx<T> px = (-ky) * tr;
+                            const cx<T> py = kx * tr;
+                            ix = mk(px.re / k2, px.im / k2);
+                            iy = mk(py.re / k2, py.im / k2);
+                        }
+                        if (is_k0) {
+                            k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fx_field] + at, fx_field, ix, sp);
+                            k0_write<T>(P.base_k0, P.cur_k0, P.next_k0, P.next[fy_field] + at, fy_field, iy, sp);
+                        } else {
+                            gstore(P.next[fx_field], at, wb * bx + wc * (jx + ix));
+                            gstore(P.next[fy_field], at, wb * by + wc * (jy + iy));
+                        }
                     }
                 }
             }
