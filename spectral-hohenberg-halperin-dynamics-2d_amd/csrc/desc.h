@@ -184,6 +184,8 @@ struct InstDev {
     int ntiles;               // inverse-column tiles = ceil(Nax/2 / C)
     int C3;                   // kx columns per forward-column workgroup
     int ntiles3;              // forward-column tiles
+    int k3_order;             // forward-column workgroup order: 1 momentum groups first in each XCD's
+                              // range (a launch of few rounds), 0 momentum group first in each tile
 };
 
 // An instance whose density reached zero stops at the last step it completed

@@ -51,6 +51,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Slot i of the n slots [start, start + n) of the forward-column pass, ng
+// groups per tile (slot l: tile l / ng, group l mod ng; the momentum group
+// last), taken momentum groups first: the slots l = ng - 1 (mod ng), then
+// the others in order.  A bijection of the range onto itself.
+__device__ __forceinline__ int heavy_first(int start, int n, int i, int ng) {
+    const int m0 = start + (ng - 1 - start % ng);   // first momentum slot >= start
+    const int nm = m0 < start + n ? (start + n - 1 - m0) / ng + 1 : 0;
+    if (i < nm) return m0 + i * ng;
+    const int t = (start / ng) * (ng - 1) + min(start % ng, ng - 1) + (i - nm);   // rank among the psi slots
+    return (t / (ng - 1)) * ng + t % (ng - 1);
+}
+
 // element (kx, y) of a T plane in the blocked layout (Desc::tlg >= 0)
 __device__ __forceinline__ uint32_t t_at(const Desc& d, int kx, int y) {
     return ((static_cast<uint32_t>(kx >> d.tlg) * d.my + y) << d.tlg) + (kx & ((1 << d.tlg) - 1));
@@ -1001,12 +1013,31 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int inst = map.inst[blk];
-    const int local = blk - map.first[inst];
+    int local = blk - map.first[inst];
     const InstDev<T>& I = insts[inst];
     if (instance_failed(I.flag)) return;
     const Desc& d = I.d;
     const StagePtrs<T>& P = ptrs[inst];
     const int C = I.C3;
+    // Workgroup order (the momentum group has five transforms and three
+    // fields to the psi groups' three and one; the dispatcher hands an XCD's
+    // workgroups out in order).  A launch of a few rounds of resident
+    // workgroups with the instance alone in it takes each XCD's range of
+    // slots momentum groups first, so the long ones start first and the short
+    // ones fill in behind them; a longer launch keeps the groups mixed (both
+    // the memory and the arithmetic pipes stay busy) with the momentum group
+    // first in each tile, so no range ends on a long workgroup.
+    if (d.ngroups > 1 && d.groups[d.ngroups - 1].kind == GROUP_MOM) {
+        if (I.k3_order == 1 && I.ntiles3 * d.ngroups == static_cast<int>(gridDim.x)) {
+            constexpr int X = 8;
+            const int b = blockIdx.x, nb = gridDim.x;
+            const int x = b % X, q = nb / X, r = nb % X;
+            local = heavy_first(x * q + (x < r ? x : r), q + (x < r ? 1 : 0), b / X, d.ngroups);
+        } else {
+            const int t = local / d.ngroups, g = local - t * d.ngroups;
+            local = t * d.ngroups + (g == 0 ? d.ngroups - 1 : g - 1);
+        }
+    }
     const cx<T>* __restrict__ ubuf = I.ubuf;
     const cx<T>* __restrict__ twy = I.twy;
     {

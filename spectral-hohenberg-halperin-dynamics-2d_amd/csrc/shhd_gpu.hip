@@ -1927,6 +1927,19 @@ struct Engine {
             lc.lds[1] = std::max(lc.lds[1], h.rows.lds);
             lc.lds[2] = std::max(lc.lds[2], h.cols3.lds3);
         }
+        // forward-column workgroup order: momentum groups first when the launch
+        // is at most four rounds of the workgroups the GPU holds at once
+        {
+            const ClassKey& K = lc.key;
+            K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb, K.c3ws);
+            int per_cu = 0, ncu = 0;
+            SHHD_HIP(&err, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k3),
+                                                                       K.c3NT, lc.lds[2]));
+            SHHD_HIP(&err, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+            const int nb3 = static_cast<int>(maps[2].size());
+            const int order = (per_cu > 0 && ncu > 0 && nb3 <= 4 * per_cu * ncu) ? 1 : 0;
+            for (auto& I : inst) I.k3_order = order;
+        }
         SHHD_HIP(&err, hipMalloc(&lc.d_inst, inst.size() * sizeof(InstDev<T>)));
         SHHD_HIP(&err, hipMemcpy(lc.d_inst, inst.data(), inst.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
         SHHD_HIP(&err, hipMalloc(&lc.d_ptrs, ptrs.size() * sizeof(StagePtrs<T>)));
