@@ -149,13 +149,7 @@ struct StageParams {
     // adds the incompressible projection pressure
     double flux_w;
     int flux_first, flux_last;
-    int noise_mode;         // NoiseMode
 };
-
-// Noise of a stage: drawn in the kernel (Euler), drawn and stored for the
-// later stages of the step (SRK3 stage 1), or read back (SRK3 stages 2, 3;
-// the draws A and B are the same for all three stages).
-enum NoiseMode : int { NOISE_DRAW = 0, NOISE_STORE = 1, NOISE_LOAD = 2 };
 
 // Flux request bits (FluxRequest, src/buffer_flux.h)
 enum FluxBits : int { FLUX_DENSITY = 1, FLUX_ORDER_PARAMETER = 2, FLUX_MOMENTUM = 4 };
@@ -177,8 +171,6 @@ struct InstDev {
     cx<T>* flux;              // step flux, compact slots [rho x, rho y, (psi x, psi y)*N,
                               // Pi xx, Pi xy, Pi yy]; null unless requested
     int flux_req;             // FluxBits
-    cx<T>* noise;             // SRK3 noise of the step: per psi group draws A, B;
-                              // momentum: A_x, A_y, B_x, B_y (fields of field_elems); null if no noise
     int C;                    // kx columns per inverse-column workgroup
     int R;                    // rows per row-pass workgroup
     int ntiles;               // inverse-column tiles = ceil(Nax/2 / C)

@@ -1063,7 +1063,6 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
     // Every mode is read and written by the one lane that owns it in all the
     // loops below (m = tid + NT * i), so the accumulation needs no atomics.
     const size_t fe = static_cast<size_t>(hxa) * nay;
-    const uint32_t fe32 = static_cast<uint32_t>(fe);
     cx<T>* fx_slot = nullptr;   // psi group: psi flux x; momentum group: rho flux x
     cx<T>* fy_slot = nullptr;
     cx<T>* pi_slot = nullptr;   // Pi xx, then Pi xy and Pi yy at +fe, +2fe
@@ -1419,24 +1418,14 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                     cx<T> sto = czero<T>();
                     if (d.psi_noise_on[a] && k2 != T(0)) {   // src/fcalculator.cc:568-584
                         // draws A and B (conjugated on the kx = 0, ky < 0 half line);
-                        // SRK3 draws them once per step: stage 1 stores them, stages 2
-                        // and 3 read them back
-                        cx<T> sa, sb = czero<T>();
-                        cx<T>* nz = I.noise ? I.noise + static_cast<size_t>(2 * a) * fe : nullptr;
-                        if (sp.noise_mode == NOISE_LOAD) {
-                            sa = gload(nz, at);
-                            sb = gload(nz, fe32 + at);
-                        } else {
-                            const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
-                            gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
-                            sa = sigma * sa;
-                            sb = sp.use_b ? sigma * sb : czero<T>();
-                            if (conj_draw) { sa = conj(sa); sb = conj(sb); }
-                            if (sp.noise_mode == NOISE_STORE) {
-                                gstore(nz, at, sa);
-                                gstore(nz, fe32 + at, sb);
-                            }
-                        }
+                        // the counter-based generator gives every SRK3 stage of a
+                        // step the same A and B, so each stage draws them itself
+                        cx<T> sa, sb;
+                        const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
+                        gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
+                        sa = sigma * sa;
+                        sb = sp.use_b ? sigma * sb : czero<T>();
+                        if (conj_draw) { sa = conj(sa); sb = conj(sb); }
                         sto = sp.use_b ? sa + beta * sb : sa;
                         if (fx_slot && sp.flux_first) {   // draw A: (i k / k^2) sigma xi, times 1/sqrt(dt)
                             const T isd = T(1.0 / d.sqrt_dt);
@@ -1489,11 +1478,7 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                         }
                     }
                     cx<T> sx = czero<T>(), sy = czero<T>();
-                    cx<T>* nzj = I.noise ? I.noise + static_cast<size_t>(2 * d.N) * fe : nullptr;
-                    if (d.j_noise_on && k2 != T(0) && sp.noise_mode == NOISE_LOAD) {
-                        sx = gload(nzj, at) + beta * gload(nzj, 2 * fe32 + at);
-                        sy = gload(nzj, fe32 + at) + beta * gload(nzj, 3 * fe32 + at);
-                    } else if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
+                    if (d.j_noise_on && k2 != T(0)) {   // src/fcalculator.cc:586-700
                         const T ss = T(d.j_sigma), se = T(d.j_sqrt_eta), sz = T(d.j_sqrt_zeta);
                         const T kxd = conj_draw ? T(0) : kx;
                         const T kyd = conj_draw ? -ky : ky;
@@ -1515,10 +1500,6 @@ __global__ __launch_bounds__(NT) void k_forward_columns(const InstDev<T>* __rest
                                 pi_slot[at] = pi_slot[at] + isd * (conj_draw ? conj(pxx) : pxx);
                                 pi_slot[at + fe] = pi_slot[at + fe] + isd * (conj_draw ? conj(pxy) : pxy);
                                 pi_slot[at + 2 * fe] = pi_slot[at + 2 * fe] + isd * (conj_draw ? conj(pyy) : pyy);
-                            }
-                            if (sp.noise_mode == NOISE_STORE) {   // draw A: sx, sy; draw B: at +2 fe, +3 fe
-                                gstore(nzj, (2 * ab) * fe32 + at, vx);
-                                gstore(nzj, (2 * ab + 1) * fe32 + at, vy);
                             }
                             if (ab == 0) { sx = vx; sy = vy; }
                             else { sx = sx + beta * vx; sy = sy + beta * vy; }

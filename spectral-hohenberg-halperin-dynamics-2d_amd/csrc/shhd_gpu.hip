@@ -448,10 +448,9 @@ struct Instance {
     cx<double>* k0[3] = {nullptr, nullptr, nullptr};
     void* tbuf = nullptr;
     void* ubuf = nullptr;
-    void* noise = nullptr;    // SRK3 noise of the current step (NoiseMode)
     void* twx = nullptr;
     void* twy = nullptr;
-    // state, k0, tbuf, ubuf, noise and the twiddle tables above are carved out
+    // state, k0, tbuf, ubuf and the twiddle tables above are carved out
     // of this one allocation (one hipMalloc and one hipFree per instance)
     void* slab = nullptr;
     std::vector<int> static_fields;
@@ -1040,24 +1039,13 @@ void apply_plans(Instance& h, const Tuning& t) {
     }
 }
 
-// SRK3 with noise in the FP32 form keeps the step's draws A and B for
-// stages 2 and 3: 2 fields per order parameter and 4 for the momentum (the
-// FP64 form draws them again)
-size_t noise_elems(const Instance& h) {
-    const Desc& d = h.desc;
-    bool on = d.j_noise_on != 0;
-    for (int a = 0; a < d.N; ++a) on = on || d.psi_noise_on[a] != 0;
-    if (h.stages != 3 || !on || !h.fp32) return 0;
-    return static_cast<size_t>(2 * d.N + 4) * h.field_elems;
-}
-
-// bytes a configuration holds on the device (state x3, work planes, noise, twiddles)
+// bytes a configuration holds on the device (state x3, work planes, twiddles)
 // The parts of an instance's one device allocation, 256-byte aligned: state
-// x3, FP64 k = 0 copies x3 (FP32 form), T planes, U planes, noise cache,
-// x and y twiddle tables.
+// x3, FP64 k = 0 copies x3 (FP32 form), T planes, U planes, x and y twiddle
+// tables.
 struct SlabParts {
-    size_t state, k0, t, u, noise, twx, twy;
-    size_t total() const { return 3 * state + 3 * k0 + t + u + noise + twx + twy; }
+    size_t state, k0, t, u, twx, twy;
+    size_t total() const { return 3 * state + 3 * k0 + t + u + twx + twy; }
 };
 SlabParts slab_parts(const Instance& h) {
     const size_t nf = static_cast<size_t>(h.desc.nfields);
@@ -1067,7 +1055,6 @@ SlabParts slab_parts(const Instance& h) {
     s.k0 = h.fp32 ? al(nf * sizeof(cx<double>)) : 0;
     s.t = al(std::max<size_t>(1, static_cast<size_t>(h.n_in)) * h.plane_elems * h.elem);
     s.u = al(std::max<size_t>(1, static_cast<size_t>(h.n_out)) * h.plane_elems * h.elem);
-    s.noise = al(noise_elems(h) * h.elem);
     s.twx = al(static_cast<size_t>(fz_tw_size(h.cfg.nx)) * h.elem);
     s.twy = al(static_cast<size_t>(fz_tw_size(h.cfg.ny)) * h.elem);
     return s;
@@ -1113,10 +1100,10 @@ shhd_status upload_twiddles(Instance* h, std::string* err) {
 
 // One device allocation per instance, carved into 256-byte aligned parts:
 // state x3 and the FP64 k = 0 copies first (cleared with one memset), then
-// the T and U planes, the noise cache and the two twiddle tables.
+// the T and U planes and the two twiddle tables.
 shhd_status allocate(Instance* h, std::string* err) {
     const SlabParts parts = slab_parts(*h);
-    const size_t sb = parts.state, kb = parts.k0, tb = parts.t, ub = parts.u, nb = parts.noise, xb = parts.twx;
+    const size_t sb = parts.state, kb = parts.k0, tb = parts.t, ub = parts.u, xb = parts.twx;
     SHHD_HIP(err, hipMalloc(&h->slab, parts.total()));
     char* p = static_cast<char*>(h->slab);
     for (int b = 0; b < 3; ++b, p += sb) h->state[b] = p;
@@ -1127,8 +1114,6 @@ shhd_status allocate(Instance* h, std::string* err) {
     p += tb;
     h->ubuf = p;
     p += ub;
-    if (nb) h->noise = p;
-    p += nb;
     h->twx = p;
     p += xb;
     h->twy = p;
@@ -1897,7 +1882,6 @@ struct Engine {
             I.flag = d_flags + lc.members[static_cast<size_t>(i)];
             I.step = d_steps + lc.members[static_cast<size_t>(i)];
             I.flux = h.flux_req ? static_cast<cx<T>*>(h.flux) : nullptr;
-            I.noise = static_cast<cx<T>*>(h.noise);
             I.flux_req = h.flux_req;
             I.C = h.cols.C;
             I.R = h.rows.R;
@@ -2150,10 +2134,10 @@ struct Engine {
                     sp.flux_w = 1.0;
                 }
                 sp.flux_first = s == 0 ? 1 : 0;
-                // SRK3 draws A and B once per step: the FP32 form keeps them for
-                // stages 2 and 3, the FP64 form draws them again (the counter-based
-                // generator gives the same values; measured faster than the cache)
-                sp.noise_mode = (lc.key.stages == 3 && lc.key.fp32) ? (s == 0 ? NOISE_STORE : NOISE_LOAD) : NOISE_DRAW;
+                // SRK3 uses draws A and B of the step at every stage; the
+                // counter-based generator gives each stage the same values, so
+                // every stage draws them (measured faster than keeping them in HBM
+                // for stages 2 and 3, in both precisions)
                 sp.flux_last = s == lc.key.stages - 1 ? 1 : 0;
                 shhd_status st = lc.key.fp32 ? launch_stage<float>(lc, k, s, sp, capturing)
                                              : launch_stage<double>(lc, k, s, sp, capturing);
