@@ -1912,14 +1912,19 @@ struct Engine {
             lc.lds[2] = std::max(lc.lds[2], h.cols3.lds3);
         }
         // forward-column workgroup order: momentum groups first when the launch
-        // is at most four rounds of the workgroups the GPU holds at once
+        // is at most four rounds of the workgroups the GPU holds at once (an
+        // occupancy query that fails keeps the mixed order; either order gives
+        // the same results)
         {
             const ClassKey& K = lc.key;
             K3Fn<T> k3 = pick_k3<T>(K.c3NT, K.c3V, K.acc, K.c3fz, K.c3nb, K.c3ws);
             int per_cu = 0, ncu = 0;
-            SHHD_HIP(&err, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k3),
-                                                                       K.c3NT, lc.lds[2]));
-            SHHD_HIP(&err, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k3), K.c3NT,
+                                                             lc.lds[2]) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+                per_cu = ncu = 0;
+                (void)hipGetLastError();   // not left for the next launch check to report
+            }
             const int nb3 = static_cast<int>(maps[2].size());
             const int order = (per_cu > 0 && ncu > 0 && nb3 <= 4 * per_cu * ncu) ? 1 : 0;
             for (auto& I : inst) I.k3_order = order;
