@@ -67,6 +67,20 @@ template <typename T>
 __device__ __forceinline__ void gstore(cx<T>* base, uint32_t i, cx<T> z) {
     gstore(reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(cx<T>))), z);
 }
+// Elements i and i + 1 (i even, base aligned to two elements) in one store
+template <typename T> struct vec4_of;
+template <> struct vec4_of<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct vec4_of<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <typename T>
+__device__ __forceinline__ void gstore2(cx<T>* base, uint32_t i, cx<T> a, cx<T> b) {
+    using V = typename vec4_of<T>::type;
+    V v;
+    v.x = a.re;
+    v.y = a.im;
+    v.z = b.re;
+    v.w = b.im;
+    *(__attribute__((address_space(1))) V*)(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(cx<T>))) = v;
+}
 #endif
 
 // ---------------------------------------------------------------------------

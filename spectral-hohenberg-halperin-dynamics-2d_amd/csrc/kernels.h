@@ -750,6 +750,27 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             const int umask = (1 << d.ulg) - 1;
             static_assert((RF & (RF - 1)) == 0, "rows per workgroup a power of two");
             const int lgb = log2p2(RF) + d.ulg;      // log2 of the rows x columns of one U block segment
+            if (sizeof(T) == 4 && RF == 1 && d.ulg >= 1 && (hxa & 1) == 0) {
+                // FP32, one row (y0 < my) in blocks of at least two columns: a
+                // lane takes the two columns kx = k, k + 1 (k even) of one block
+                // segment (adjacent in U, one 16-byte store each plane; in FP64
+                // the 32-byte stores touch twice the segments per instruction
+                // and measured 20 % slower)
+                const uint32_t yrow = static_cast<uint32_t>(y0) << d.ulg;
+                for (int k = 2 * tid; k < hxa; k += 2 * NT) {
+                    const cx<T> z0 = res[lp(k)];
+                    const cx<T> z1 = res[lp(k + 1)];
+                    const cx<T> m0 = res[lp(k == 0 ? 0 : FL - k)];
+                    const cx<T> m1 = res[lp(FL - 1 - k)];
+                    const uint32_t at = ((static_cast<uint32_t>(k >> d.ulg) * my) << d.ulg) + yrow + (k & umask);
+                    gstore2(ua, at, mk(T(0.5) * (z0.re + m0.re), T(0.5) * (z0.im - m0.im)),
+                            mk(T(0.5) * (z1.re + m1.re), T(0.5) * (z1.im - m1.im)));
+                    if (has_b)
+                        gstore2(ub, at, mk(T(0.5) * (z0.im + m0.im), T(0.5) * (m0.re - z0.re)),
+                                mk(T(0.5) * (z1.im + m1.im), T(0.5) * (m1.re - z1.re)));
+                }
+                continue;
+            }
             for (int idx = tid; idx < nk; idx += NT) {
                 const int kb = idx >> lgb;
                 const int rem = idx - (kb << lgb);
