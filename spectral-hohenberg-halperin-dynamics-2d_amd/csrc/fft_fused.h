@@ -170,11 +170,16 @@ __device__ __forceinline__ Tw8<T> load_tw8(const cx<T>* __restrict__ pass, int k
         typedef float f4 __attribute__((ext_vector_type(4)));
         const __attribute__((address_space(1))) f4* p = (const __attribute__((address_space(1))) f4*)(pass);
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
+        for (int h = 0; h < 3; ++h) {
             const f4 v = p[h * NS + k];
             r.w[2 * h] = mk(v.x, v.y);
-            if (h < 3) r.w[2 * h + 1] = mk(v.z, v.w);
+            r.w[2 * h + 1] = mk(v.z, v.w);
         }
+        // the padded pair of q = 7: only its first twiddle is loaded (a
+        // 16-byte load whose unused half is overwritten must complete first)
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 v7 = *(const __attribute__((address_space(1))) f2*)(p + 3 * NS + k);
+        r.w[6] = mk(v7.x, v7.y);
     } else {
 #pragma unroll
         for (int q = 0; q < 7; ++q) r.w[q] = gload(pass + q * NS + k);
