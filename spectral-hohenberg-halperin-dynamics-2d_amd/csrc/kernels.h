@@ -547,6 +547,8 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
         using MI = FzMap<NT, FL, RF, R0>;
         constexpr int B3 = PPT / 3;                 // radix-3 butterflies per lane
         static_assert(FzMap<NT, FL, RF, 3>::NB == B3 * NT, "whole radix-3 butterflies per lane");
+        // RF == 1: a launch has one workgroup per row of each instance, so
+        // every workgroup's row is inside the grid (no row masks)
         // The inverse transforms' last radix-8 / radix-3 passes and the forward
         // transforms' radix-3 / first radix-8 passes exchange across lanes
         // (FzGrp map of the lane's six points) in FP64 below 1024 lanes.  The
@@ -577,8 +579,11 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             // 3/2-padded rows (Nax/2 = FL/3), where each of the lane's R0
             // points is known at compile time to be padding, kept or
             // mirrored, except the two runs that straddle FL/3 and 2FL/3.
-            // BLK: kx-blocked T (Desc::tlg), else row-major.
-            auto load_points = [&]<bool TH, bool BLK>() {
+            // BLK: kx-blocked T (Desc::tlg), else row-major.  LIN (TH and
+            // BLK, 2^tlg dividing M): the kept point j + qM of a lane sits at
+            // t_at(j) + qM My and the mirrored one FL - qM - j at
+            // (FL - qM) My + t_at(-j), a lane base plus a uniform offset.
+            auto load_points = [&]<bool TH, bool BLK, bool LIN>() {
                 constexpr int M = MI::M;
                 constexpr int H = FL / 3;
                 const int hx = TH ? H : hxa;
@@ -588,19 +593,33 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                     if (!MI::valid(id)) continue;
                     const int y = y0 + MI::seq(id);
                     const int j = MI::bfly(id);
-                    const int yy = y < my ? y : 0;
+                    const int yy = (RF == 1 || y < my) ? y : 0;
                     const uint32_t row = static_cast<uint32_t>(yy * hxa);
+                    int kbase = 0, mbase = 0;
+                    if constexpr (LIN) {
+                        kbase = static_cast<int>(t_at(d, j, yy));
+                        mbase = (((-j) >> d.tlg) * my << d.tlg) + (yy << d.tlg) + ((-j) & ((1 << d.tlg) - 1));
+                    }
                     [&]<int... Q>(std::integer_sequence<int, Q...>) {
                         ([&] {
                             constexpr int q = Q;
                             constexpr bool dead = TH && q * M >= H && q * M + M - 1 <= FL - H;
+                            constexpr bool kept = TH && q * M + M - 1 < H;           // all k < Nax/2
+                            constexpr bool mirrored = TH && q * M > FL - H;          // all k > FL - Nax/2
                             if constexpr (!dead) {
                                 // one load per field: kx = k (k < Nax/2) or the
                                 // mirror kx = FL - k (k > FL - Nax/2) of the Hermitian row
                                 const int k = j + q * M;
-                                const bool mirror = k > FL - hx;
-                                const int kk = mirror ? FL - k : (k < hx ? k : 0);
-                                const uint32_t at = BLK ? t_at(d, kk, yy) : row + kk;
+                                uint32_t at;
+                                if constexpr (LIN && kept) {
+                                    at = static_cast<uint32_t>(kbase + q * M * my);
+                                } else if constexpr (LIN && mirrored) {
+                                    at = static_cast<uint32_t>(mbase + (FL - q * M) * my);
+                                } else {
+                                    const bool mirror = mirrored || (!kept && k > FL - hx);
+                                    const int kk = kept ? k : (mirror ? FL - k : (k < hx ? k : 0));
+                                    at = BLK ? t_at(d, kk, yy) : row + kk;
+                                }
                                 v[b][q] = gload(ta, at);
                                 vb[b][q] = gload(tb, at);   // tb = ta when there is no second field
                             }
@@ -611,7 +630,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 for (int b = 0; b < MI::B; ++b) {
                     const int id = lane + NT * b;
                     if (!MI::valid(id)) continue;
-                    const bool row_ok = y0 + MI::seq(id) < my;
+                    const bool row_ok = RF == 1 || y0 + MI::seq(id) < my;
                     const int j = MI::bfly(id);
                     [&]<int... Q>(std::integer_sequence<int, Q...>) {
                         ([&] {
@@ -643,11 +662,11 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
                 }
             };
             if (d.tlg < 0) {
-                if (3 * hxa == FL) load_points.template operator()<true, false>();
-                else load_points.template operator()<false, false>();
+                if (3 * hxa == FL) load_points.template operator()<true, false, false>();
+                else load_points.template operator()<false, false, false>();
             } else {
-                if (3 * hxa == FL) load_points.template operator()<true, true>();
-                else load_points.template operator()<false, true>();
+                if (3 * hxa == FL && (MI::M & ((1 << d.tlg) - 1)) == 0) load_points.template operator()<true, true, true>();
+                else load_points.template operator()<false, true, false>();
             }
             fz_first<T, NT, FL, RF, false, NBUF>(bf, v, +1, tid);
             if constexpr (PAIR) fz_rest_to_regs_pair<T, NT, FL, RF, NBUF>(bf, w, twx, +1, tid);
@@ -711,7 +730,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             for (int i = 0; i < PPT; ++i) {
                 const int id = tid + NT * (i / 3);
                 const int row = PAIR ? FzGrp<FL>(tid).s : FzMap<NT, FL, RF, 3>::seq(id);
-                if (F.make_velocity(i, d.velocity_compressible != 0, r0) && y0 + row < my) atomicOr(flag, 1);
+                if (F.make_velocity(i, d.velocity_compressible != 0, r0) && (RF == 1 || y0 + row < my)) atomicOr(flag, 1);
             }
         }
 
