@@ -183,12 +183,18 @@ def measured(cfg, steps, warmup, ics, workload_key, precision):
             "step_GBs": sum(kb[k] * kt[k][1] for k in kt) / max(steps, 1) / (el / steps) / 1e9}
 
 
+def note(msg):
+    """A progress line on stderr (the result is the one JSON line on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def north_star(steps, warmup, tune):
     """Model-H N=2 at 4096^2 (3/2 -> 6144^2), FP64 and the FP32 form: the
     configuration the path's throughput target is quoted on."""
     out = {}
     for prec in (64, 32):
         cfg, desc = workload("model_h_4096", prec)
+        note(f"4096^2 fp{prec}")
         shapes = shhd.autotune(cfg, steps=2) if tune else None
         ics = device_initial_conditions("model_h_4096")
         m = measured(cfg, steps, warmup, ics, "model_h_4096", prec)
@@ -346,7 +352,9 @@ def main():
                   "row_transform": rt, "plane_block": pb, "inverse_plane_block": tb, "forward_column_waves": fw,
                   "row_waves": rw, "given": True}
     else:
+        note(f"{args.workload} fp{args.precision}: launch shapes")
         tuning = shhd.autotune(cfg, steps=4) if not args.no_tune else None
+    note(f"{args.workload} fp{args.precision}: {args.steps} steps")
     ics = device_initial_conditions(args.workload)
     m = measured(cfg, args.steps, args.warmup, ics, args.workload, args.precision)
     info = m["info"]
@@ -378,19 +386,24 @@ def main():
     }
     if args.precision == 64 and not args.no_fp32:
         cfg32, _ = workload(args.workload, 32)
+        note(f"{args.workload} fp32")
         tuning32 = shhd.autotune(cfg32, steps=4) if not args.no_tune else None
         m32 = measured(cfg32, args.steps, args.warmup, ics, args.workload, 32)
         result["fp32"] = {"value": m32["value"], "ms_per_step": m32["ms_per_step"],
                           "speedup_vs_fp64": m32["value"] / value, "launch_shapes": tuning32,
-                          "roofline": m32["roofline"], "error_vs_fp64": fp32_error(cfg, ics)}
+                          "roofline": m32["roofline"], "kernels": m32["kernels"],
+                          "error_vs_fp64": fp32_error(cfg, ics)}
     if not args.no_north_star and args.workload == "model_h_1024" and args.precision == 64:
         result["north_star"] = north_star(args.north_star_steps, max(2, args.warmup // 4), not args.no_tune)
     if not args.no_serve:
+        note("ensemble")
         result["ensemble"] = ensemble_bench(args.ensemble)
+        note("served stream")
         result["served"] = served_bench(args.serve_jobs)
         result["p50_latency_s"] = result["served"]["p50_s"]
         result["p99_latency_s"] = result["served"]["p99_s"]
     if not args.no_cpu:
+        note("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(result), flush=True)
 
