@@ -49,9 +49,31 @@ struct ContribCoef {
     double a[2][4];
     double b[2][2];
     double f[3][3];
+    // the same coefficients rounded to FP32 (the FP32 form reads these: no
+    // conversion per mode)
+    float af[2][4];
+    float bf[2][2];
+    float ff[3][3];
     // nonzero entries: a[r][t] bit 4r + t, b[r][t] bit 8 + 2r + t, f[t][u] bit 12 + 3t + u
     unsigned nz;
 };
+
+// coefficient r, t of a contribution in the arithmetic type T
+template <typename T>
+__host__ __device__ __forceinline__ T coef_a(const ContribCoef& c, int r, int t) {
+    if constexpr (sizeof(T) == 4) return c.af[r][t];
+    else return c.a[r][t];
+}
+template <typename T>
+__host__ __device__ __forceinline__ T coef_b(const ContribCoef& c, int r, int t) {
+    if constexpr (sizeof(T) == 4) return c.bf[r][t];
+    else return c.b[r][t];
+}
+template <typename T>
+__host__ __device__ __forceinline__ T coef_f(const ContribCoef& c, int t, int u) {
+    if constexpr (sizeof(T) == 4) return c.ff[t][u];
+    else return c.f[t][u];
+}
 
 struct Group {
     int kind;        // GROUP_PSI or GROUP_MOM

@@ -224,7 +224,15 @@ struct RowFields {
     // rho -> 1/rho (compressible) or 1/rho0; true when the density is zero
     __device__ __forceinline__ bool make_velocity(int i, bool compressible, T r0) {
         const T r = compressible ? rho[i] : r0;
-        rho[i] = T(1) / r;
+        if constexpr (sizeof(T) == 4) {
+            // FP32 form: the hardware reciprocal (1 ulp) and one Newton step
+            // instead of the correctly rounded division sequence (r = 0 gives
+            // a non-finite value and the zero-density flag, as the division)
+            const float e = __builtin_amdgcn_rcpf(r);
+            rho[i] = __builtin_fmaf(e, __builtin_fmaf(-r, e, 1.0f), e);
+        } else {
+            rho[i] = T(1) / r;
+        }
         return compressible && r == T(0);
     }
 
@@ -1160,17 +1168,17 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
         auto poly = [&](int r) {
             const unsigned m = nz >> (4 * r);
             T p = T(0);
-            if (m & 1u) p = p + T(cf.a[r][0]);
-            if (m & 2u) p = p + T(cf.a[r][1]) * (kx * kx);
-            if (m & 4u) p = p + T(cf.a[r][2]) * (ky * ky);
-            if (m & 8u) p = p + T(cf.a[r][3]) * (kx * ky);
+            if (m & 1u) p = p + coef_a<T>(cf, r, 0);
+            if (m & 2u) p = p + coef_a<T>(cf, r, 1) * (kx * kx);
+            if (m & 4u) p = p + coef_a<T>(cf, r, 2) * (ky * ky);
+            if (m & 8u) p = p + coef_a<T>(cf, r, 3) * (kx * ky);
             return p;
         };
         auto lin = [&](int r) {
             const unsigned m = nz >> (8 + 2 * r);
             T p = T(0);
-            if (m & 1u) p = p + T(cf.b[r][0]) * kx;
-            if (m & 2u) p = p + T(cf.b[r][1]) * ky;
+            if (m & 1u) p = p + coef_b<T>(cf, r, 0) * kx;
+            if (m & 2u) p = p + coef_b<T>(cf, r, 1) * ky;
             return p;
         };
         if (nz & 0x00Fu) acc0[i] = acc0[i] + poly(0) * v;
@@ -1185,7 +1193,7 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
             for (int t = 0; t < 3; ++t) {
                 cx<T>* dst = psi_group ? (t == 0 ? fx_slot : (t == 1 ? fy_slot : nullptr)) : pi_slot + t * fe;
                 if (dst == nullptr || ((nz >> (12 + 3 * t)) & 7u) == 0) continue;
-                const cx<T> fv = T(cf.f[t][0]) * v + (T(cf.f[t][1]) * kx + T(cf.f[t][2]) * ky) * mv;
+                const cx<T> fv = coef_f<T>(cf, t, 0) * v + (coef_f<T>(cf, t, 1) * kx + coef_f<T>(cf, t, 2) * ky) * mv;
                 dst[at] = dst[at] + fw * fv;
             }
         }
@@ -1467,7 +1475,7 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
                         // the counter-based generator gives every SRK3 stage of a
                         // step the same A and B, so each stage draws them itself
                         cx<T> sa, sb;
-                        const T sigma = T(sqrt(d.psi_noise_pref[a] * static_cast<double>(k2)));
+                        const T sigma = sqrt(T(d.psi_noise_pref[a]) * k2);   // in T: FP32 needs no FP64 square root
                         gauss_quad<T>(d, mode_id, static_cast<uint32_t>(a), sp, sa, sb);   // draws A and B
                         sa = sigma * sa;
                         sb = sp.use_b ? sigma * sb : czero<T>();

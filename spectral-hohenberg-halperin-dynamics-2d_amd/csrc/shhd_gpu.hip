@@ -566,6 +566,12 @@ ContribCoef contrib_coef(const Desc& d, OutItem it) {
     for (int t = 0; t < 3; ++t)
         for (int u = 0; u < 3; ++u)
             if (c.f[t][u] != 0.0) c.nz |= 1u << (12 + 3 * t + u);
+    for (int r = 0; r < 2; ++r) {
+        for (int t = 0; t < 4; ++t) c.af[r][t] = static_cast<float>(c.a[r][t]);
+        for (int t = 0; t < 2; ++t) c.bf[r][t] = static_cast<float>(c.b[r][t]);
+    }
+    for (int t = 0; t < 3; ++t)
+        for (int u = 0; u < 3; ++u) c.ff[t][u] = static_cast<float>(c.f[t][u]);
     return c;
 }
 
