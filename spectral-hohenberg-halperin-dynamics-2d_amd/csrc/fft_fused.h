@@ -109,9 +109,17 @@ struct FzBufs {
     }
 };
 
-// LDS slot of element i of a buffer in the padded (lp) or the plain layout
-template <bool PAD>
-__device__ __forceinline__ int fz_slot(int i) { return PAD ? lp(i) : i; }
+constexpr bool fz_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// LDS layouts of a transform buffer: plain (slot i), padded (lp(i): one slot
+// per 16) or padded per group of G consecutive elements (slot i + i / G).
+constexpr int kLayPlain = 0, kLayPad = 1, kLayGroup = 2;
+template <int LAY, int G>
+__device__ __forceinline__ int fz_slot(int i) {
+    if constexpr (LAY == kLayPad) return lp(i);
+    else if constexpr (LAY == kLayGroup) return i + i / G;
+    else return i;
+}
 
 // Layout of the buffer pass P writes: plain after a radix-3 first pass (its
 // stride-3 writes are conflict-free unpadded and conflict 2-way padded; the
@@ -119,20 +127,33 @@ __device__ __forceinline__ int fz_slot(int i) { return PAD ? lp(i) : i; }
 // after an inner pass of sub-length 24 (its writes are runs of 24 consecutive
 // elements over consecutive lanes: conflict-free unpadded for 16-byte
 // elements, at most 2-way for 8-byte ones, where the padded slot would cost
-// four integer operations per output), padded after every other pass
-// (stride 8, 4, 2 writes) and after the last pass (the callers read the
-// result in the padded layout).
-constexpr bool fz_pad_after(int L, bool r3_first, int P) {
-    return !(P == 0 && fz_radix(L, r3_first, 0) == 3) &&
-           !(fz_ns(L, r3_first, P) == 24 && P + 1 < fz_npass(L, r3_first));
+// four integer operations per output); padded per group of ns R elements
+// after a pass whose sub-length ns is not a power of two (the radix-8 pass at
+// ns = 3 after a radix-3 first pass: butterfly group m writes the run
+// [m ns R, (m + 1) ns R), so every output sits at the group's padded base
+// plus a constant, where the lp slot of o + p ns costs four integer
+// operations per output; its unpadded stride-3 writes conflict 3-way, the
+// group-padded ones at most 2-way on a few lanes, and the next pass, whose
+// butterfly stride is a multiple of ns R, reads at a lane base plus
+// constants); padded after every other pass (stride 8, 4, 2 writes) and after
+// the last pass (the callers read the result in the padded layout).
+constexpr int fz_layout_after(int L, bool r3_first, int P) {
+    if (P == 0 && fz_radix(L, r3_first, 0) == 3) return kLayPlain;
+    if (fz_ns(L, r3_first, P) == 24 && P + 1 < fz_npass(L, r3_first)) return kLayPlain;
+    const int ns = fz_ns(L, r3_first, P), g = ns * fz_radix(L, r3_first, P);
+    if (P + 1 < fz_npass(L, r3_first) && !fz_pow2(ns) && ns % kPad != 0 && g >= kPad &&
+        (L / fz_radix(L, r3_first, P + 1)) % g == 0)
+        return kLayGroup;
+    return kLayPad;
 }
-
-constexpr bool fz_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// group of the layout pass P writes (ns R of that pass: the next pass's ns)
+constexpr int fz_group_after(int L, bool r3_first, int P) { return fz_ns(L, r3_first, P) * fz_radix(L, r3_first, P); }
 
 // Stockham outputs of the butterflies in v to LDS.
-template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
+template <typename T, int NT, int L, int NSEQ, int R, int NS, int LAY = kLayPad>
 __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R], int tid) {
     using Mp = FzMap<NT, L, NSEQ, R>;
+    static_assert(LAY != kLayGroup || L % (NS * R) == 0, "whole groups per sequence");
 #pragma unroll
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
@@ -145,14 +166,20 @@ __device__ __forceinline__ void fz_write(cx<T>* __restrict__ out, cx<T> (&v)[FzM
         // block), lp(o + p NS) = lp(o) + p NS + (p NS >> kPadShift) for every
         // butterfly: o = (multiple of NS R) + k with k < NS, so no output run
         // crosses a pad slot it does not account for; the per-output offsets
-        // are then compile-time constants.
-        if constexpr (PAD && (NS % kPad == 0 || (fz_pow2(NS) && fz_pow2(R) && (NSEQ == 1 || L % kPad == 0)))) {
+        // are then compile-time constants.  In the group layout the run
+        // o + p NS (p < R) stays inside group o / (NS R) for the same reason.
+        if constexpr (LAY == kLayGroup) {
+            const int o0 = o + o / (NS * R);
+#pragma unroll
+            for (int p = 0; p < R; ++p) out[o0 + p * NS] = v[b][p];
+        } else if constexpr (LAY == kLayPad &&
+                             (NS % kPad == 0 || (fz_pow2(NS) && fz_pow2(R) && (NSEQ == 1 || L % kPad == 0)))) {
             const int o0 = lp(o);
 #pragma unroll
             for (int p = 0; p < R; ++p) out[o0 + p * NS + ((p * NS) >> kPadShift)] = v[b][p];
         } else {
 #pragma unroll
-            for (int p = 0; p < R; ++p) out[fz_slot<PAD>(o + p * NS)] = v[b][p];
+            for (int p = 0; p < R; ++p) out[fz_slot<LAY, NS * R>(o + p * NS)] = v[b][p];
         }
     }
 }
@@ -229,10 +256,13 @@ __device__ __forceinline__ auto fz_twiddles(const cx<T>* __restrict__ tw, int si
 
 // Butterfly inputs from LDS, twiddled (twiddles loaded beforehand),
 // transformed in registers.
-template <typename T, int NT, int L, int NSEQ, int R, int NS, bool PAD = true>
+// LAY: the layout the previous pass wrote (kLayGroup: groups of NS, this
+// pass's sub-length, which divides the butterfly stride M).
+template <typename T, int NT, int L, int NSEQ, int R, int NS, int LAY = kLayPad>
 __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)[FzMap<NT, L, NSEQ, R>::B][R],
                                         const FzTw<T, NT, L, NSEQ, R, NS>& twv, int sign, int tid) {
     using Mp = FzMap<NT, L, NSEQ, R>;
+    static_assert(LAY != kLayGroup || (Mp::M % NS == 0 && L % NS == 0), "group layout: whole groups per stride");
 #pragma unroll
     for (int b = 0; b < Mp::B; ++b) {
         const int id = tid + NT * b;
@@ -240,13 +270,17 @@ __device__ __forceinline__ void fz_read(const cx<T>* __restrict__ in, cx<T> (&v)
         const int s = Mp::seq(id);
         const int j = Mp::bfly(id);
         const int a = s * L + j;
-        if constexpr (PAD && Mp::M % kPad == 0) {
+        if constexpr (LAY == kLayGroup) {
+            const int a0 = a + a / NS;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b][q] = in[a0 + q * (Mp::M + Mp::M / NS)];
+        } else if constexpr (LAY == kLayPad && Mp::M % kPad == 0) {
             const int a0 = lp(a);
 #pragma unroll
             for (int q = 0; q < R; ++q) v[b][q] = in[a0 + q * (Mp::M + Mp::M / kPad)];
         } else {
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[b][q] = in[fz_slot<PAD>(a + q * Mp::M)];
+            for (int q = 0; q < R; ++q) v[b][q] = in[fz_slot<LAY, NS>(a + q * Mp::M)];
         }
     }
 #pragma unroll
@@ -274,7 +308,7 @@ __device__ __forceinline__ void fz_first(FzBufs<T, NBUF>& bf,
     for (int b = 0; b < Mp::B; ++b)
         if (Mp::valid(tid + NT * b)) dft_small<T, R>(v[b], sign);
     bf.before_write();
-    fz_write<T, NT, L, NSEQ, R, 1, fz_pad_after(L, R3F, 0)>(bf.wr(), v, tid);
+    fz_write<T, NT, L, NSEQ, R, 1, fz_layout_after(L, R3F, 0)>(bf.wr(), v, tid);
     bf.flip();
 }
 
@@ -286,9 +320,9 @@ __device__ __forceinline__ void fz_pass(FzBufs<T, NBUF>& bf, const cx<T>* __rest
     cx<T> v[FzMap<NT, L, NSEQ, R>::B][R];
     const auto twv = fz_twiddles<T, NT, L, NSEQ, R3F, P>(tw, sign, tid);
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P - 1)>(bf.rd(), v, twv, sign, tid);
+    fz_read<T, NT, L, NSEQ, R, NS, fz_layout_after(L, R3F, P - 1)>(bf.rd(), v, twv, sign, tid);
     bf.before_write();
-    fz_write<T, NT, L, NSEQ, R, NS, fz_pad_after(L, R3F, P)>(bf.wr(), v, tid);
+    fz_write<T, NT, L, NSEQ, R, NS, fz_layout_after(L, R3F, P)>(bf.wr(), v, tid);
     bf.flip();
 }
 
@@ -319,7 +353,7 @@ __device__ __forceinline__ void fz_rest_to_regs(
     constexpr int RL = fz_radix(L, R3F, NP - 1), NSL = fz_ns(L, R3F, NP - 1);
     const auto twv = fz_twiddles<T, NT, L, NSEQ, R3F, NP - 1>(tw, sign, tid);
     __syncthreads();
-    fz_read<T, NT, L, NSEQ, RL, NSL, fz_pad_after(L, R3F, NP - 2)>(bf.rd(), v, twv, sign, tid);
+    fz_read<T, NT, L, NSEQ, RL, NSL, fz_layout_after(L, R3F, NP - 2)>(bf.rd(), v, twv, sign, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,7 +449,8 @@ __device__ __forceinline__ void fz_rest_to_regs_pair(FzBufs<T, NBUF>& bf, cx<T> 
     constexpr int NP = fz_npass(L, false);
     constexpr int K = L / 24;
     constexpr int M8 = L / 8;                       // radix-8 butterflies per sequence
-    constexpr bool PAD = fz_pad_after(L, false, NP - 3);
+    constexpr int LAY = fz_layout_after(L, false, NP - 3);
+    static_assert(LAY != kLayGroup, "radix 3 last: power-of-two sub-lengths before the pair");
     asm volatile("" : "+v"(tid));
     fz_passes<T, NT, L, NSEQ, false, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 3>{});
     const FzGrp<L> G(tid);
@@ -431,13 +466,13 @@ __device__ __forceinline__ void fz_rest_to_regs_pair(FzBufs<T, NBUF>& bf, cx<T> 
     if (act) {
         const cx<T>* __restrict__ in = bf.rd();
         const int a = G.s * L + K * G.r + G.k;
-        if constexpr (PAD && M8 % kPad == 0) {
+        if constexpr (LAY == kLayPad && M8 % kPad == 0) {
             const int a0 = lp(a);
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] = in[a0 + q * (M8 + M8 / kPad)];
         } else {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = in[fz_slot<PAD>(a + q * M8)];
+            for (int q = 0; q < 8; ++q) v[q] = in[fz_slot<LAY, 1>(a + q * M8)];
         }
         if constexpr (fz_tw8_planes<T, NT, 8>()) mul_tw8<T>(v, t8);
         else mul_twiddles<T, 8>(v, t8);
@@ -497,10 +532,14 @@ __device__ __forceinline__ void fz_first_pair(FzBufs<T, NBUF>& bf, cx<T> (&v)[2]
     bf.before_write();
     if (act) {
         // Stockham output of butterfly j8 = 3k + r at ns = 3: 24 k + r + 3 p
+        // (group layout: the run of group s L / 24 + k at its padded base)
+        static_assert(fz_layout_after(L, true, 1) == kLayGroup && fz_group_after(L, true, 1) == 24,
+                      "the ns = 3 pass writes groups of 24");
         cx<T>* __restrict__ out = bf.wr();
         const int o = G.s * L + 24 * G.k + G.r;
+        const int o0 = o + o / 24;
 #pragma unroll
-        for (int p = 0; p < 8; ++p) out[fz_slot<fz_pad_after(L, true, 1)>(o + 3 * p)] = x[p];
+        for (int p = 0; p < 8; ++p) out[o0 + 3 * p] = x[p];
     }
     bf.flip();
 }
