@@ -9,6 +9,8 @@ elsewhere), reference-layout states in and out through the C-ABI:
 - Model-H N=4 (the any-order row pass built for up to four order parameters,
   which the served stream's N = 4 jobs run) at 256^2 for 20 steps and at
   1024^2 for 3 steps;
+- Model-H with N = 3, 5 and 8 distinct order parameters (the builds for up to
+  four and up to eight) at 128^2 for 10 steps and N = 8 at 512^2 for 3;
 - Model-H N=2 at 2048^2 (3072-point fused builds, the served stream's
   largest grid), 3 steps;
 - Model-H N=2 at 4096^2 (6144-point fused builds, the north-star workload),
@@ -92,6 +94,32 @@ def test_model_h_four_order_parameters(na, steps):
         cfg = model_h_n(na, 4, precision)
         got, info = gpu_run(cfg, st, steps)
         assert info.num_inverse_fields == 7 and info.num_forward_fields == 17
+        compare(cfg, st, got, ref, tol)
+
+
+def model_h_distinct(na, n, precision):
+    """N order parameters with a different mobility, quadratic coefficients
+    and u for each, so a mix-up of the index alpha anywhere shows."""
+    cfg = model_h(na, precision, noise=False)
+    return dataclasses.replace(cfg, num_order_parameters=n, mobility=[1.0 + 0.1 * a for a in range(n)],
+                               mu_k0=[-1.0 + 0.05 * a for a in range(n)], mu_k2=[1.0 - 0.05 * a for a in range(n)],
+                               phi4_u=[1.0 + 0.2 * a for a in range(n)])
+
+
+@pytest.mark.parametrize("na,n,steps", [(128, 3, 10), (128, 5, 10), (128, 8, 10), (512, 8, 3)])
+def test_model_h_up_to_eight_order_parameters(na, n, steps):
+    """The row-pass builds for up to four (N = 3: one slot unused) and up to
+    eight order parameters (N = 5, 8; the largest the C-ABI accepts), staged
+    rows of 192 points and fused rows of 768."""
+    cfg64 = model_h_distinct(na, n, 64)
+    st = synthetic_state(cfg64, seed=20 + n)
+    banner(f"oracle: model-H N={n} {na}^2, {steps} steps")
+    ref = oracle_run(cfg64, st, steps)
+    for precision, tol in ((64, 1e-10), (32, 1e-5)):
+        banner(f"model-H N={n} {na}^2 {steps} steps vs oracle fp{precision}")
+        cfg = model_h_distinct(na, n, precision)
+        got, info = gpu_run(cfg, st, steps)
+        assert info.num_inverse_fields == n + 3 and info.num_forward_fields == 3 * n + 5
         compare(cfg, st, got, ref, tol)
 
 
