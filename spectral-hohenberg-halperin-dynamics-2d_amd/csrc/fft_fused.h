@@ -194,18 +194,21 @@ template <typename T, int NS>
 __device__ __forceinline__ Tw8<T> load_tw8(const cx<T>* __restrict__ pass, int k, int sign) {
     Tw8<T> r;
     if constexpr (sizeof(T) == 4) {
+        // a uniform base and a 32-bit lane byte offset (the table is far
+        // below 4 GiB), as gload does
         typedef float f4 __attribute__((ext_vector_type(4)));
-        const __attribute__((address_space(1))) f4* p = (const __attribute__((address_space(1))) f4*)(pass);
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const char* base = reinterpret_cast<const char*>(pass);
+        const uint32_t kb = static_cast<uint32_t>(k) * 16u;
 #pragma unroll
         for (int h = 0; h < 3; ++h) {
-            const f4 v = p[h * NS + k];
+            const f4 v = *(const __attribute__((address_space(1))) f4*)(base + (kb + uint32_t(h * NS * 16)));
             r.w[2 * h] = mk(v.x, v.y);
             r.w[2 * h + 1] = mk(v.z, v.w);
         }
         // the padded pair of q = 7: only its first twiddle is loaded (a
         // 16-byte load whose unused half is overwritten must complete first)
-        typedef float f2 __attribute__((ext_vector_type(2)));
-        const f2 v7 = *(const __attribute__((address_space(1))) f2*)(p + 3 * NS + k);
+        const f2 v7 = *(const __attribute__((address_space(1))) f2*)(base + (kb + uint32_t(3 * NS * 16)));
         r.w[6] = mk(v7.x, v7.y);
     } else {
 #pragma unroll
@@ -304,6 +307,7 @@ __device__ __forceinline__ void fz_first(FzBufs<T, NBUF>& bf,
     // an opaque copy of the lane index: the constant-folded LDS offsets of
     // each call are formed here, not hoisted out of the caller's loops
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
 #pragma unroll
     for (int b = 0; b < Mp::B; ++b)
         if (Mp::valid(tid + NT * b)) dft_small<T, R>(v[b], sign);
@@ -337,6 +341,7 @@ __device__ __forceinline__ void fz_passes(FzBufs<T, NBUF>& bf, const cx<T>* __re
 template <typename T, int NT, int L, int NSEQ, bool R3F, int NBUF>
 __device__ __forceinline__ void fz_rest_to_lds(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign, int tid) {
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, fz_npass(L, R3F) - 1>{});
 }
 
@@ -349,6 +354,7 @@ __device__ __forceinline__ void fz_rest_to_regs(
     const cx<T>* __restrict__ tw, int sign, int tid) {
     constexpr int NP = fz_npass(L, R3F);
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     fz_passes<T, NT, L, NSEQ, R3F, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 2>{});
     constexpr int RL = fz_radix(L, R3F, NP - 1), NSL = fz_ns(L, R3F, NP - 1);
     const auto twv = fz_twiddles<T, NT, L, NSEQ, R3F, NP - 1>(tw, sign, tid);
@@ -452,6 +458,7 @@ __device__ __forceinline__ void fz_rest_to_regs_pair(FzBufs<T, NBUF>& bf, cx<T> 
     constexpr int LAY = fz_layout_after(L, false, NP - 3);
     static_assert(LAY != kLayGroup, "radix 3 last: power-of-two sub-lengths before the pair");
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     fz_passes<T, NT, L, NSEQ, false, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, NP - 3>{});
     const FzGrp<L> G(tid);
     const bool act = G.r < 3;
@@ -508,6 +515,7 @@ __device__ __forceinline__ void fz_first_pair(FzBufs<T, NBUF>& bf, cx<T> (&v)[2]
                                               int sign, int tid) {
     static_assert(fz_pair_ok<L, true>() && NT * 6 == NSEQ * L, "radix-3 / radix-8 pair shape");
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     const FzGrp<L> G(tid);
     const bool act = G.r < 3;
     const int k8 = act ? G.r : 0;   // twiddle index of radix-8 butterfly 3k + r (ns = 3)
@@ -554,6 +562,7 @@ template <typename T, int NT, int L, int NSEQ, int NBUF>
 __device__ __forceinline__ void fz_rest_to_lds_after_pair(FzBufs<T, NBUF>& bf, const cx<T>* __restrict__ tw, int sign,
                                                           int tid) {
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     fz_passes_from2<T, NT, L, NSEQ, NBUF>(bf, tw, sign, tid, std::make_integer_sequence<int, fz_npass(L, true) - 2>{});
 }
 

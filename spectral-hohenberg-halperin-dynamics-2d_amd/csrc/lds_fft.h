@@ -304,6 +304,7 @@ __device__ __forceinline__ void lds_fft_ct(cx<T>* buf, int nseq, const cx<T>* __
     // call are then formed here, not hoisted out of the caller's loops and
     // held live across them
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < NT);   // the range the copy hides
     lds_fft_ct_passes<T, NT, VPT, L>(buf, nseq, tw, sign, tid, std::make_integer_sequence<int, ct_npass(L)>{});
 }
 
