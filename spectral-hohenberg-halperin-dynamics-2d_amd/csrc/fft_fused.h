@@ -212,7 +212,7 @@ __device__ __forceinline__ Tw8<T> load_tw8(const cx<T>* __restrict__ pass, int k
         r.w[6] = mk(v7.x, v7.y);
     } else {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) r.w[q] = gload(pass + q * NS + k);
+        for (int q = 0; q < 7; ++q) r.w[q] = gload(pass, static_cast<uint32_t>(q * NS + k));
     }
 #pragma unroll
     for (int q = 0; q < 7; ++q)

@@ -579,6 +579,7 @@ __global__ __launch_bounds__(NT) void k_rows(const InstDev<T>* __restrict__ inst
             // per transform, not held live across the loop
             int lane = tid;
             asm volatile("" : "+v"(lane));
+            __builtin_assume(lane >= 0 && lane < NT);   // the range the copy hides
             // every load is issued before any is used, at an index that is
             // always inside the row (kx = 0 for the zero-padded points and
             // row 0 past the last row), and the padding is zeroed afterwards
@@ -1240,6 +1241,7 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
             const cx<T>* __restrict__ src = ubuf + static_cast<size_t>(g.contrib[ci]) * plane;
             int lane0 = tidf;
             asm volatile("" : "+v"(lane0));
+            __builtin_assume(lane0 >= 0 && lane0 < NTF);   // the range the copy hides
 #pragma unroll
             for (int b = 0; b < M0::B; ++b) {
                 const int id = lane0 + NTF * b;
@@ -1274,6 +1276,7 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
             // recomputed per contribution instead of held live across the transform
             int lane = tidf;
             asm volatile("" : "+v"(lane));
+            __builtin_assume(lane >= 0 && lane < NTF);   // the range the copy hides
 #pragma unroll
             for (int b = 0; b < B3; ++b) {
                 const int s = seq_of(b, lane);

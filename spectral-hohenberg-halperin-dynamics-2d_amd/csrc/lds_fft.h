@@ -108,7 +108,7 @@ struct Twiddles {
 template <typename T, int R>
 __device__ __forceinline__ Twiddles<T, R> load_twiddles(const cx<T>* __restrict__ tw, int t, int sign) {
     auto ld = [&](int e) {
-        cx<T> w = gload(tw + e);
+        cx<T> w = gload(tw, static_cast<uint32_t>(e));
         if (sign > 0) w.im = -w.im;
         return w;
     };
