@@ -12,7 +12,7 @@ with kBT = 1e-3 (in-kernel Philox), so every stage draws its stochastic terms.
 resident in HBM; `roofline` is the dominant kernel's algorithmic HBM bytes per
 launch over its mean launch time (HIP events on the solver's stream);
 `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same
-workload on the host cores.  `north_star` measures the same model at 4096^2
+workload on the host cores.  `model_h_4096` measures the same model at 4096^2
 (3/2 -> 6144^2), FP64 and the FP32 form, each with its own roofline.
 """
 import argparse
@@ -94,8 +94,9 @@ def initial_state(cfg, seed=2):
 
 def device_initial_conditions(name):
     """The `set` commands of the configurations that name reference styles,
-    evaluated on the device: C1 equilibrium/gaussian density and momentum
-    (seed 1, kBT 1e-4, rho0 1, cT 1); C3 the k^4 exp(-2 (k/8)^2) spectrum,
+    evaluated on the device: 128^2 compressible Navier-Stokes, equilibrium/gaussian
+    density and momentum (seed 1, kBT 1e-4, rho0 1, cT 1); 4096^2 incompressible, the
+    k^4 exp(-2 (k/8)^2) spectrum,
     urms 1, seed 3, rho0 1.  None: the synthetic state of initial_state()."""
     IC = shhd.initial_condition
     if name == "ns_128":
@@ -188,7 +189,7 @@ def note(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def north_star(steps, warmup, tune):
+def model_h_4096(steps, warmup, tune):
     """Model-H N=2 at 4096^2 (3/2 -> 6144^2), FP64 and the FP32 form: the
     configuration the path's throughput target is quoted on."""
     out = {}
@@ -334,8 +335,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the ensemble and served-stream measurements")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=256)
-    ap.add_argument("--no-north-star", action="store_true", help="skip the 4096^2 FP64/FP32 measurement")
-    ap.add_argument("--north-star-steps", type=int, default=60)
+    ap.add_argument("--no-4096", action="store_true", help="skip the 4096^2 FP64/FP32 measurement")
+    ap.add_argument("--steps-4096", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration of this path fits one MI355X and runs are "
@@ -393,8 +394,8 @@ def main():
                           "speedup_vs_fp64": m32["value"] / value, "launch_shapes": tuning32,
                           "roofline": m32["roofline"], "kernels": m32["kernels"],
                           "error_vs_fp64": fp32_error(cfg, ics)}
-    if not args.no_north_star and args.workload == "model_h_1024" and args.precision == 64:
-        result["north_star"] = north_star(args.north_star_steps, max(2, args.warmup // 4), not args.no_tune)
+    if not args.no_4096 and args.workload == "model_h_1024" and args.precision == 64:
+        result["model_h_4096"] = model_h_4096(args.steps_4096, max(2, args.warmup // 4), not args.no_tune)
     if not args.no_serve:
         note("ensemble")
         result["ensemble"] = ensemble_bench(args.ensemble)

@@ -18,7 +18,7 @@ structure-factor sums over the run.  Sampling error of one snapshot of E_K
 is sqrt(E_K kBT) ~ 1 %; the averages of a few hundred correlated samples are
 held to 3 %.
 
-C1 (compressible Navier-Stokes 128^2, cT = 1, eta = zeta = 0.1, kBT = 1e-4,
+The 128^2 compressible Navier-Stokes configuration (cT = 1, eta = zeta = 0.1, kBT = 1e-4,
 dt = 0.05, 2000 steps) must keep a flat S_rho_rho(k) = rho0 kBT / cT^2.
 """
 import numpy as np
@@ -97,8 +97,8 @@ def test_equipartition_energies(mode, precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_c1_flat_density_structure_factor(precision):
-    banner(f"C1 compressible NS 128^2 S_rho_rho flat, fp{precision}")
+def test_ns128_flat_density_structure_factor(precision):
+    banner(f"compressible NS 128^2 S_rho_rho flat, fp{precision}")
     kbt = 1e-4
     cfg = shhd.SolverConfig(nx_active=128, ny_active=128, dealias="3/2", lx=128.0, ly=128.0, dt=0.05,
                             pressure_coefficient=1.0, eta=0.1, zeta=0.1, momentum_advection=True,

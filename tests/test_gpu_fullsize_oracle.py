@@ -1,4 +1,4 @@
-"""GPU against the FP64 oracle where the served and north-star workloads run.
+"""GPU against the FP64 oracle where the served and the 4096^2 Model-H workloads run.
 
 Deterministic runs (kBT = 0; the oracle draws the reference's mt19937_64
 stream, the device a Philox stream, so noise is compared in distribution
@@ -13,7 +13,7 @@ elsewhere), reference-layout states in and out through the C-ABI:
   four and up to eight) at 128^2 for 10 steps and N = 8 at 512^2 for 3;
 - Model-H N=2 at 2048^2 (3072-point fused builds, the served stream's
   largest grid), 3 steps;
-- Model-H N=2 at 4096^2 (6144-point fused builds, the north-star workload),
+- Model-H N=2 at 4096^2 (6144-point fused builds, the largest single-grid workload),
   one step.
 
 The oracle runs on all host threads it is given; at 1024^2 a step takes about
