@@ -1228,7 +1228,7 @@ void k_forward_columns(const InstDev<T>* __restrict__ insts,
         cx<T>* const gbuf = lds + static_cast<size_t>(grp) * NBUF * lp_size(CF * FZL);
         FzBufs<T, NBUF> bf{gbuf, gbuf + lp_size(CF * FZL), 0};
         // the first butterflies of contribution ci, straight from U (zeros for
-        // ci = none: a group with no product left this round)
+        // ci = none: a group with no product left in this iteration)
         const int ncon = g.ncontrib;
         auto load_contrib = [&](int ci, cx<T> (&v)[M0::B][R0]) {
             if (ci >= ncon) {
