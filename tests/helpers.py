@@ -93,6 +93,11 @@ def flux_configs():
         out[f"quiescent_{tag}"] = shhd.SolverConfig(
             **base, **phi4(1), integrator=integ, dt=0.004, mode=shhd.MODE_QUIESCENT,
             noise_order_parameter=True, kBT=0.02)
+    # three order parameters: the flux slots of an order parameter beyond the
+    # first two, through the row pass built for up to four
+    out["model_h3_srk3"] = shhd.SolverConfig(
+        **base, **phi4(3), dt=0.004, pressure_coefficient=1.0, eta=0.5, zeta=0.3, momentum_advection=True,
+        order_parameter_advection=True, noise_momentum=True, noise_order_parameter=True, kBT=1e-4)
     return out
 
 
