@@ -13,6 +13,7 @@ elsewhere), reference-layout states in and out through the C-ABI:
   four and up to eight) at 128^2 for 10 steps and N = 8 at 512^2 for 3;
 - Model-H N=2 at 2048^2 (3072-point fused builds, the served stream's
   largest grid), 3 steps;
+- Model-H N=2 at 1024^2 with forward Euler, 4 steps;
 - Model-H N=2 at 4096^2 (6144-point fused builds, the largest single-grid workload),
   one step.
 
@@ -132,6 +133,23 @@ def test_model_h_2048():
         banner(f"model-H N=2 2048^2 3 steps vs oracle fp{precision}")
         cfg = model_h(2048, precision, noise=False)
         got, info = gpu_run(cfg, st, 3)
+        assert info.row_fused > 0 and info.forward_column_fused > 0
+        compare(cfg, st, got, ref, tol)
+
+
+def test_model_h_1024_euler():
+    """Forward Euler through the fused 1536-point builds: one stage a step,
+    whose update has no base-state term (a graph of three steps, then one
+    step outside it)."""
+    euler = lambda p: dataclasses.replace(model_h(1024, p, noise=False), integrator=shhd.INTEGRATOR_EULER)  # noqa: E731
+    cfg64 = euler(64)
+    st = synthetic_state(cfg64, seed=15)
+    banner("oracle: model-H N=2 1024^2, 4 Euler steps")
+    ref = oracle_run(cfg64, st, 4)
+    for precision, tol in ((64, 1e-10), (32, 1e-5)):
+        banner(f"model-H N=2 1024^2 4 Euler steps vs oracle fp{precision}")
+        cfg = euler(precision)
+        got, info = gpu_run(cfg, st, 4)
         assert info.row_fused > 0 and info.forward_column_fused > 0
         compare(cfg, st, got, ref, tol)
 
