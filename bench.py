@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
-    ap.add_argument("--launch-shapes", default="", metavar="C1,C3,NT[,CT,RT,PB[,TB[,FW[,RW]]]]",
+    ap.add_argument("--launch-shapes", default="", metavar="CI,CF,NT[,CT,RT,PB[,TB[,FW[,RW]]]]",
                     help="use these launch shapes (inverse tile, forward tile, row lanes, column and row "
                          "transform choice, log2 row-to-column plane block, column-to-row plane block, "
                          "forward-column and row wave groups, as in shhd_tuning) instead of tuning")
